@@ -1,0 +1,87 @@
+"""ctypes binding of libhtamd.so (the C ABI declared in include/htamd.h).
+
+The library is built in-tree by ``python -m holotomocupy_amd.build``.  There is
+no CPU fallback: if the library or a GPU is missing, every operator raises.
+"""
+import ctypes
+import os
+
+import torch
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libhtamd.so")
+
+c_i64 = ctypes.c_int64
+c_int = ctypes.c_int
+c_dbl = ctypes.c_double
+c_vp = ctypes.c_void_p
+c_fp = ctypes.POINTER(ctypes.c_float)
+c_dp = ctypes.POINTER(ctypes.c_double)
+
+# name -> argtypes; every entry point returns int status (0 = OK)
+SIGNATURES = {
+    "htamd_prop_plan_create": [c_i64, c_i64, c_i64, c_dbl, c_dbl, c_dp, ctypes.POINTER(c_vp)],
+    "htamd_prop_apply": [c_vp, c_vp, c_vp, c_i64, c_i64, c_int, c_vp],
+    "htamd_prop_plan_destroy": [c_vp],
+}
+
+_lib = None
+
+
+class HtamdError(RuntimeError):
+    """Raised when a libhtamd entry point returns a non-zero status."""
+
+
+def load():
+    """Load libhtamd.so once and attach the ctypes signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HtamdError(
+            f"{LIB_PATH} is missing: build it with `python -m holotomocupy_amd.build` "
+            "(there is no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.htamd_last_error.restype = ctypes.c_char_p
+    lib.htamd_last_error.argtypes = []
+    lib.htamd_abi_version.restype = c_int
+    lib.htamd_abi_version.argtypes = []
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = c_int
+    _lib = lib
+    return lib
+
+
+def check(status, what):
+    if status != 0:
+        msg = load().htamd_last_error().decode(errors="replace")
+        raise HtamdError(f"{what} failed ({status}): {msg}")
+
+
+def call(name, *args):
+    check(getattr(load(), name)(*args), name)
+
+
+def require_gpu():
+    if not torch.cuda.is_available():
+        raise HtamdError("holotomocupy_amd needs a ROCm GPU; none is visible (no CPU fallback)")
+
+
+def stream_ptr(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return c_vp(s.cuda_stream)
+
+
+def ptr(t):
+    return c_vp(t.data_ptr())
+
+
+def as_device(x, dtype, device):
+    """Return a contiguous tensor of `dtype` on `device` (copies only if needed)."""
+    if not isinstance(x, torch.Tensor):
+        x = torch.as_tensor(x)
+    if x.device != device or x.dtype != dtype:
+        x = x.to(device=device, dtype=dtype)
+    return x.contiguous()

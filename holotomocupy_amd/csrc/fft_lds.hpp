@@ -1,0 +1,125 @@
+// LDS-resident Stockham FFT for power-of-two lengths on CDNA4 (wave64).
+//
+// One transform of length N is carried by T = N/E threads; each thread owns E
+// complex values per stage and performs E/R radix-R butterflies.  The data
+// lives in LDS between stages (in-place, two workgroup barriers per stage);
+// twiddles come from a float2 table tw[m] = exp(-2*pi*i*m/N) generated in FP64
+// on the host.  Output is in natural order (Stockham autosort).
+#pragma once
+#include "htamd_common.hpp"
+
+namespace htamd {
+
+template <bool INV>
+__device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw, int m) {
+  float2 w = tw[m];
+  if (INV) w.y = -w.y;
+  return w;
+}
+
+// In-register DFTs.  Forward: X[k] = sum_r a[r] exp(-2 pi i r k / R).
+template <bool INV>
+__device__ __forceinline__ void dft2(float2& a0, float2& a1) {
+  float2 t = a0;
+  a0 = cadd(t, a1);
+  a1 = csub(t, a1);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+  float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+  float2 t2 = cadd(a1, a3), t3 = mul_mi<INV>(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft8(float2* a) {
+  constexpr float h = 0.70710678118654752440f;
+  float2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+  float2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+  dft4<INV>(e0, e1, e2, e3);
+  dft4<INV>(o0, o1, o2, o3);
+  // o_k *= exp(-+ 2 pi i k / 8)
+  o1 = INV ? make_float2(h * (o1.x - o1.y), h * (o1.x + o1.y))
+           : make_float2(h * (o1.x + o1.y), h * (o1.y - o1.x));
+  o2 = mul_mi<INV>(o2);
+  o3 = INV ? make_float2(-h * (o3.x + o3.y), h * (o3.x - o3.y))
+           : make_float2(h * (o3.y - o3.x), -h * (o3.x + o3.y));
+  a[0] = cadd(e0, o0); a[4] = csub(e0, o0);
+  a[1] = cadd(e1, o1); a[5] = csub(e1, o1);
+  a[2] = cadd(e2, o2); a[6] = csub(e2, o2);
+  a[3] = cadd(e3, o3); a[7] = csub(e3, o3);
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2* a) {
+  if constexpr (R == 2) {
+    dft2<INV>(a[0], a[1]);
+  } else if constexpr (R == 4) {
+    dft4<INV>(a[0], a[1], a[2], a[3]);
+  } else if constexpr (R == 8) {
+    dft8<INV>(a);
+  } else {
+    static_assert(R == 2 || R == 4 || R == 8, "unsupported radix");
+  }
+}
+
+// One Stockham stage: span NS (product of radices done so far), radix R.
+template <int N, int E, int R, int NS, bool INV>
+__device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* __restrict__ tw) {
+  constexpr int T = N / E;
+  constexpr int NB = E / R;
+  float2 a[NB][R];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + b * T;
+#pragma unroll
+    for (int r = 0; r < R; ++r) a[b][r] = s[j + r * (N / R)];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + b * T;
+    const int k = j & (NS - 1);
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) a[b][r] = cmul(a[b][r], twiddle<INV>(tw, r * k * (N / (NS * R))));
+    }
+    dft<R, INV>(a[b]);
+    const int base = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) s[base + r * NS] = a[b][r];
+  }
+  __syncthreads();
+}
+
+template <int N, int E, int NS, bool INV>
+__device__ __forceinline__ void stockham(float2* s, int t, const float2* __restrict__ tw) {
+  if constexpr (NS < N) {
+    constexpr int REM = N / NS;
+    constexpr int R = REM >= E ? E : REM;
+    stockham_stage<N, E, R, NS, INV>(s, t, tw);
+    stockham<N, E, NS * R, INV>(s, t, tw);
+  }
+}
+
+// Transform s[0..N) in place; every thread of the workgroup must call this
+// (it contains barriers).  t in [0, N/E) is the thread's index within this FFT.
+template <int N, int E, bool INV>
+__device__ __forceinline__ void fft_lds(float2* s, int t, const float2* __restrict__ tw) {
+  static_assert((N & (N - 1)) == 0 && N >= 2, "N must be a power of two");
+  static_assert(E <= N && (E == 2 || E == 4 || E == 8), "E must be 2, 4 or 8");
+  stockham<N, E, 1, INV>(s, t, tw);
+}
+
+// Makhoul reordering that turns a length-N DCT-II into a length-N FFT:
+// natural index i -> FFT input slot (even i -> i/2, odd i -> N-1-i/2).
+template <int N>
+__device__ __forceinline__ int dct_slot(int i) {
+  return (i & 1) ? (N - 1 - (i >> 1)) : (i >> 1);
+}
+
+}  // namespace htamd

@@ -1,0 +1,196 @@
+// Separable Fresnel propagator D / DT for gfx950.
+//
+// The reference builds D as crop(IFFT2(FFT2(sympad(psi)) * fker)) on a
+// 2nz x 2n grid (src/holotomocupy/propagation.py:70-112).  The transfer
+// function and the symmetric pad / crop factor per axis, and a symmetric
+// pad followed by a length-2n FFT is a DCT-II, so each axis reduces to
+//     D_x = C^-1 diag(lambda_k) C,   lambda_k = exp(-i pi lambda z (k/(2 n vs))^2)
+// with C the length-n DCT-II.  C is evaluated through a length-n complex FFT
+// (Makhoul reordering); the post-twiddle of C, the chirp and the pre-twiddle
+// of C^-1 collapse into one 2x2 map on the bin pair (k, n-k):
+//     U_k   = A_k V_k + B_k V_{n-k}
+//     U_n-k = C_k V_k + A_k V_{n-k}
+// with A_k = (l_k + l_{n-k})/2n, B_k = (l_k - l_{n-k}) e^{+i pi k/n}/2n,
+// C_k = (l_k - l_{n-k}) e^{-i pi k/n}/2n, tables built in FP64 on the host.
+// DT uses the tables of conj(lambda).  Each axis is one HBM pass: read the
+// field, FFT in LDS, pair map, inverse FFT in LDS, write the field.
+#include "fft_lds.hpp"
+#include "htamd_internal.hpp"
+
+namespace htamd {
+
+// Pair map on the FFT output held in LDS (see file comment).  Table layout:
+// tab[k] = A_k, tab[H+k] = B_k, tab[2H+k] = C_k (H = N/2), except that the two
+// self-paired bins use tab[0] = l_0/N and tab[H] = l_{N/2}/N.
+template <int N, int E>
+__device__ __forceinline__ void dct_pair_map(float2* s, int t, const float2* __restrict__ tab) {
+  constexpr int T = N / E;
+  constexpr int H = N / 2;
+#pragma unroll
+  for (int q = 0; q < E / 2; ++q) {
+    const int k = t + q * T;
+    if (k == 0) {
+      s[0] = cmul(tab[0], s[0]);
+      s[H] = cmul(tab[H], s[H]);
+    } else {
+      const float2 v1 = s[k], v2 = s[N - k];
+      const float2 a = tab[k], b = tab[H + k], c = tab[2 * H + k];
+      s[k] = cadd(cmul(a, v1), cmul(b, v2));
+      s[N - k] = cadd(cmul(c, v1), cmul(a, v2));
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int E>
+__device__ __forceinline__ void fresnel_1d(float2* s, int t, const float2* __restrict__ tw,
+                                           const float2* __restrict__ tab) {
+  fft_lds<N, E, false>(s, t, tw);
+  dct_pair_map<N, E>(s, t, tab);
+  fft_lds<N, E, true>(s, t, tw);
+}
+
+constexpr int kRowThreads = 256;
+
+// Rows of length N, ROWS = 256/(N/E) rows per workgroup.
+template <int N, int E>
+__global__ __launch_bounds__(kRowThreads) void fresnel_rows_k(const float2* in, float2* out,
+                                                              int64_t nrows,
+                                                              const float2* __restrict__ tw,
+                                                              const float2* __restrict__ tab) {
+  constexpr int T = N / E;
+  constexpr int ROWS = kRowThreads / T;
+  __shared__ float2 lds[ROWS * N];
+  const int r = threadIdx.x / T, t = threadIdx.x % T;
+  const int64_t row = (int64_t)blockIdx.x * ROWS + r;
+  const bool valid = row < nrows;
+  float2* s = lds + r * N;
+  const float2* src = in + row * N;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = t + e * T;
+    s[dct_slot<N>(i)] = valid ? src[i] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fresnel_1d<N, E>(s, t, tw, tab);
+  if (valid) {
+    float2* dst = out + row * N;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = t + e * T;
+      dst[i] = s[dct_slot<N>(i)];
+    }
+  }
+}
+
+// Columns of length N (= nz) of a [batch, N, n] field, C adjacent columns per
+// workgroup, in place.  Workgroups are mapped XCD-contiguously so neighbouring
+// column tiles (which share cache lines) run under the same L2.
+template <int N, int E, int C>
+__global__ __launch_bounds__(C*(N / E)) void fresnel_cols_k(float2* data, int n,
+                                                            int tiles_per_field,
+                                                            const float2* __restrict__ tw,
+                                                            const float2* __restrict__ tab) {
+  constexpr int T = N / E;
+  constexpr int NT = C * T;
+  constexpr int LS = N + 1;  // padded LDS column stride (bank spread)
+  __shared__ float2 lds[C * LS];
+  const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t b = tile / tiles_per_field;
+  const int x0 = (int)(tile % tiles_per_field) * C;
+  float2* base = data + b * (int64_t)N * n + x0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int idx = threadIdx.x + e * NT;
+    const int c = idx % C, y = idx / C;
+    lds[c * LS + dct_slot<N>(y)] = base[(int64_t)y * n + c];
+  }
+  __syncthreads();
+  const int cc = threadIdx.x / T, t = threadIdx.x % T;
+  fresnel_1d<N, E>(lds + cc * LS, t, tw, tab);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int idx = threadIdx.x + e * NT;
+    const int c = idx % C, y = idx / C;
+    base[(int64_t)y * n + c] = lds[c * LS + dct_slot<N>(y)];
+  }
+}
+
+bool fft_size_supported(int64_t n) {
+  return n >= kFftMin && n <= kFftMax && (n & (n - 1)) == 0;
+}
+
+template <int N>
+static hipError_t rows_n(const float2* in, float2* out, int64_t nrows, const float2* tw,
+                         const float2* tab, hipStream_t st) {
+  constexpr int E = 8;
+  constexpr int ROWS = kRowThreads / (N / E);
+  const int64_t grid = (nrows + ROWS - 1) / ROWS;
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL((fresnel_rows_k<N, E>), dim3((unsigned)grid), dim3(kRowThreads), 0, st, in,
+                     out, nrows, tw, tab);
+  return hipGetLastError();
+}
+
+hipError_t launch_fresnel_rows(int64_t n, const float2* in, float2* out, int64_t nrows,
+                               const float2* tw, const float2* tab, hipStream_t st) {
+  switch (n) {
+    case 8: return rows_n<8>(in, out, nrows, tw, tab, st);
+    case 16: return rows_n<16>(in, out, nrows, tw, tab, st);
+    case 32: return rows_n<32>(in, out, nrows, tw, tab, st);
+    case 64: return rows_n<64>(in, out, nrows, tw, tab, st);
+    case 128: return rows_n<128>(in, out, nrows, tw, tab, st);
+    case 256: return rows_n<256>(in, out, nrows, tw, tab, st);
+    case 512: return rows_n<512>(in, out, nrows, tw, tab, st);
+    case 1024: return rows_n<1024>(in, out, nrows, tw, tab, st);
+    case 2048: return rows_n<2048>(in, out, nrows, tw, tab, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int N, int C>
+static hipError_t cols_nc(int64_t n, float2* data, int64_t batch, const float2* tw,
+                          const float2* tab, hipStream_t st) {
+  constexpr int E = 8;
+  const int64_t tiles_per_field = n / C;
+  const int64_t grid = batch * tiles_per_field;
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL((fresnel_cols_k<N, E, C>), dim3((unsigned)grid), dim3(C * (N / E)), 0, st,
+                     data, (int)n, (int)tiles_per_field, tw, tab);
+  return hipGetLastError();
+}
+
+// Columns per workgroup: as many as fit 1024 threads and ~64 KB of LDS, and
+// divide the row length n (n is even, so C = 2 always divides).
+template <int N>
+static hipError_t cols_n(int64_t n, float2* data, int64_t batch, const float2* tw,
+                         const float2* tab, hipStream_t st) {
+  constexpr int T = N / 8;
+  constexpr int CT = 1024 / T;
+  constexpr int CL = 8192 / N;
+  constexpr int CMAX0 = CT < CL ? CT : CL;
+  constexpr int CMAX = CMAX0 > 16 ? 16 : CMAX0;
+  if (CMAX >= 16 && n % 16 == 0) return cols_nc<N, (CMAX >= 16 ? 16 : 1)>(n, data, batch, tw, tab, st);
+  if (CMAX >= 8 && n % 8 == 0) return cols_nc<N, (CMAX >= 8 ? 8 : 1)>(n, data, batch, tw, tab, st);
+  if (CMAX >= 4 && n % 4 == 0) return cols_nc<N, (CMAX >= 4 ? 4 : 1)>(n, data, batch, tw, tab, st);
+  if (n % 2 == 0) return cols_nc<N, 2>(n, data, batch, tw, tab, st);
+  return cols_nc<N, 1>(n, data, batch, tw, tab, st);
+}
+
+hipError_t launch_fresnel_cols(int64_t nz, int64_t n, float2* data, int64_t batch,
+                               const float2* tw, const float2* tab, hipStream_t st) {
+  switch (nz) {
+    case 8: return cols_n<8>(n, data, batch, tw, tab, st);
+    case 16: return cols_n<16>(n, data, batch, tw, tab, st);
+    case 32: return cols_n<32>(n, data, batch, tw, tab, st);
+    case 64: return cols_n<64>(n, data, batch, tw, tab, st);
+    case 128: return cols_n<128>(n, data, batch, tw, tab, st);
+    case 256: return cols_n<256>(n, data, batch, tw, tab, st);
+    case 512: return cols_n<512>(n, data, batch, tw, tab, st);
+    case 1024: return cols_n<1024>(n, data, batch, tw, tab, st);
+    case 2048: return cols_n<2048>(n, data, batch, tw, tab, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace htamd

@@ -1,0 +1,25 @@
+// Internal host-side interfaces between the C ABI (capi.hip) and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+namespace htamd {
+
+void set_error(const std::string& msg);
+
+// ---- Fresnel propagator (fresnel.hip) -------------------------------------
+// Smallest / largest power-of-two transform length handled by the LDS FFT.
+constexpr int64_t kFftMin = 8;
+constexpr int64_t kFftMax = 2048;
+bool fft_size_supported(int64_t n);
+
+// Row pass: nrows rows of length n, out = D_x(in) row by row (in may equal out).
+// tab: pair table (3*(n/2) float2) of the 1-D operator, tw: twiddles (n float2).
+hipError_t launch_fresnel_rows(int64_t n, const float2* in, float2* out, int64_t nrows,
+                               const float2* tw, const float2* tab, hipStream_t st);
+// Column pass, in place: data [batch, nz, n], columns of length nz.
+hipError_t launch_fresnel_cols(int64_t nz, int64_t n, float2* data, int64_t batch,
+                               const float2* tw, const float2* tab, hipStream_t st);
+
+}  // namespace htamd
