@@ -2,20 +2,18 @@
 //
 // One transform of length N is carried by T = N/E threads; each thread owns E
 // complex values per stage and performs E/R radix-R butterflies.  The data
-// lives in LDS between stages (in-place, two workgroup barriers per stage);
-// twiddles come from a float2 table tw[m] = exp(-2*pi*i*m/N) generated in FP64
-// on the host.  Output is in natural order (Stockham autosort).
+// lives in LDS between stages (in-place, two workgroup barriers per stage).
+//
+// Twiddles: each thread needs one base twiddle w = exp(-2 pi i k/(NS R)) per
+// butterfly and stage (NS > 1); they are read once per kernel from a table
+// tw[m] = exp(-2 pi i m/N) generated in FP64 on the host, held in registers
+// (FftTw), and the powers w^2..w^(R-1) are formed with at most three complex
+// products.  The inverse transform uses the conjugates.  Output is in natural
+// order (Stockham autosort), unnormalised.
 #pragma once
 #include "htamd_common.hpp"
 
 namespace htamd {
-
-template <bool INV>
-__device__ __forceinline__ float2 twiddle(const float2* __restrict__ tw, int m) {
-  float2 w = tw[m];
-  if (INV) w.y = -w.y;
-  return w;
-}
 
 // In-register DFTs.  Forward: X[k] = sum_r a[r] exp(-2 pi i r k / R).
 template <bool INV>
@@ -67,9 +65,71 @@ __device__ __forceinline__ void dft(float2* a) {
   }
 }
 
+// ---- compile-time stage schedule -------------------------------------------
+template <int N, int E, int NS>
+constexpr int stage_radix() {
+  return (N / NS) >= E ? E : (N / NS);
+}
+// number of base twiddles a thread holds for stages starting at span NS
+template <int N, int E, int NS = 1>
+constexpr int tw_count() {
+  if constexpr (NS >= N) {
+    return 0;
+  } else {
+    constexpr int R = stage_radix<N, E, NS>();
+    return (NS > 1 ? E / R : 0) + tw_count<N, E, NS * R>();
+  }
+}
+
+template <int N, int E>
+struct FftTw {
+  static constexpr int K = tw_count<N, E>() > 0 ? tw_count<N, E>() : 1;
+  float2 w[K];
+};
+
+template <int N, int E, int NS, int OFF>
+__device__ __forceinline__ void load_tw_rec(float2* w, int t, const float2* __restrict__ tw) {
+  if constexpr (NS < N) {
+    constexpr int R = stage_radix<N, E, NS>();
+    constexpr int T = N / E;
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int b = 0; b < E / R; ++b) {
+        const int k = (t + b * T) & (NS - 1);
+        w[OFF + b] = tw[k * (N / (NS * R))];
+      }
+    }
+    load_tw_rec<N, E, NS * R, OFF + (NS > 1 ? E / R : 0)>(w, t, tw);
+  }
+}
+
+// Load this thread's base twiddles (t in [0, N/E)).  Issue early: the loads
+// overlap with whatever the kernel does before the first transform.
+template <int N, int E>
+__device__ __forceinline__ void fft_load_twiddles(FftTw<N, E>& r, int t, const float2* __restrict__ tw) {
+  load_tw_rec<N, E, 1, 0>(r.w, t, tw);
+}
+
+// powers p[r] = w^r, r = 1..R-1 (p[0] unused), products of depth <= 3
+template <int R, bool INV>
+__device__ __forceinline__ void tw_powers(float2 w, float2* p) {
+  if (INV) w.y = -w.y;
+  p[1] = w;
+  if constexpr (R > 2) {
+    p[2] = cmul(w, w);
+    p[3] = cmul(p[2], w);
+  }
+  if constexpr (R > 4) {
+    p[4] = cmul(p[2], p[2]);
+    p[5] = cmul(p[4], w);
+    p[6] = cmul(p[3], p[3]);
+    p[7] = cmul(p[4], p[3]);
+  }
+}
+
 // One Stockham stage: span NS (product of radices done so far), radix R.
 template <int N, int E, int R, int NS, bool INV>
-__device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* __restrict__ tw) {
+__device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* w) {
   constexpr int T = N / E;
   constexpr int NB = E / R;
   float2 a[NB][R];
@@ -85,8 +145,10 @@ __device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* _
     const int j = t + b * T;
     const int k = j & (NS - 1);
     if constexpr (NS > 1) {
+      float2 p[R];
+      tw_powers<R, INV>(w[b], p);
 #pragma unroll
-      for (int r = 1; r < R; ++r) a[b][r] = cmul(a[b][r], twiddle<INV>(tw, r * k * (N / (NS * R))));
+      for (int r = 1; r < R; ++r) a[b][r] = cmul(a[b][r], p[r]);
     }
     dft<R, INV>(a[b]);
     const int base = (j - k) * R + k;
@@ -96,23 +158,23 @@ __device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* _
   __syncthreads();
 }
 
-template <int N, int E, int NS, bool INV>
-__device__ __forceinline__ void stockham(float2* s, int t, const float2* __restrict__ tw) {
+template <int N, int E, int NS, int OFF, bool INV>
+__device__ __forceinline__ void stockham(float2* s, int t, const float2* w) {
   if constexpr (NS < N) {
-    constexpr int REM = N / NS;
-    constexpr int R = REM >= E ? E : REM;
-    stockham_stage<N, E, R, NS, INV>(s, t, tw);
-    stockham<N, E, NS * R, INV>(s, t, tw);
+    constexpr int R = stage_radix<N, E, NS>();
+    stockham_stage<N, E, R, NS, INV>(s, t, w + OFF);
+    stockham<N, E, NS * R, OFF + (NS > 1 ? E / R : 0), INV>(s, t, w);
   }
 }
 
 // Transform s[0..N) in place; every thread of the workgroup must call this
-// (it contains barriers).  t in [0, N/E) is the thread's index within this FFT.
+// (it contains barriers).  t in [0, N/E) is the thread's index within this FFT
+// and tw its registers from fft_load_twiddles with the same t.
 template <int N, int E, bool INV>
-__device__ __forceinline__ void fft_lds(float2* s, int t, const float2* __restrict__ tw) {
+__device__ __forceinline__ void fft_lds(float2* s, int t, const FftTw<N, E>& tw) {
   static_assert((N & (N - 1)) == 0 && N >= 2, "N must be a power of two");
   static_assert(E <= N && (E == 2 || E == 4 || E == 8), "E must be 2, 4 or 8");
-  stockham<N, E, 1, INV>(s, t, tw);
+  stockham<N, E, 1, 0, INV>(s, t, tw.w);
 }
 
 // Makhoul reordering that turns a length-N DCT-II into a length-N FFT:

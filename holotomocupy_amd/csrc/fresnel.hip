@@ -43,7 +43,7 @@ __device__ __forceinline__ void dct_pair_map(float2* s, int t, const float2* __r
 }
 
 template <int N, int E>
-__device__ __forceinline__ void fresnel_1d(float2* s, int t, const float2* __restrict__ tw,
+__device__ __forceinline__ void fresnel_1d(float2* s, int t, const FftTw<N, E>& tw,
                                            const float2* __restrict__ tab) {
   fft_lds<N, E, false>(s, t, tw);
   dct_pair_map<N, E>(s, t, tab);
@@ -66,13 +66,15 @@ __global__ __launch_bounds__(kRowThreads) void fresnel_rows_k(const float2* in, 
   const bool valid = row < nrows;
   float2* s = lds + r * N;
   const float2* src = in + row * N;
+  FftTw<N, E> twr;
+  fft_load_twiddles<N, E>(twr, t, tw);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = t + e * T;
     s[dct_slot<N>(i)] = valid ? src[i] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fresnel_1d<N, E>(s, t, tw, tab);
+  fresnel_1d<N, E>(s, t, twr, tab);
   if (valid) {
     float2* dst = out + row * N;
 #pragma unroll
@@ -99,6 +101,9 @@ __global__ __launch_bounds__(C*(N / E)) void fresnel_cols_k(float2* data, int n,
   const int64_t b = tile / tiles_per_field;
   const int x0 = (int)(tile % tiles_per_field) * C;
   float2* base = data + b * (int64_t)N * n + x0;
+  const int cc = threadIdx.x / T, t = threadIdx.x % T;
+  FftTw<N, E> twr;
+  fft_load_twiddles<N, E>(twr, t, tw);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int idx = threadIdx.x + e * NT;
@@ -106,8 +111,7 @@ __global__ __launch_bounds__(C*(N / E)) void fresnel_cols_k(float2* data, int n,
     lds[c * LS + dct_slot<N>(y)] = base[(int64_t)y * n + c];
   }
   __syncthreads();
-  const int cc = threadIdx.x / T, t = threadIdx.x % T;
-  fresnel_1d<N, E>(lds + cc * LS, t, tw, tab);
+  fresnel_1d<N, E>(lds + cc * LS, t, twr, tab);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int idx = threadIdx.x + e * NT;
