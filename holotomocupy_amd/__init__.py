@@ -8,3 +8,4 @@ gfx950 HIP kernels behind the C ABI of libhtamd.so.  There is no CPU fallback.
 __version__ = "0.1.0"
 
 from .propagation import Propagation  # noqa: F401
+from .shift import Shift  # noqa: F401

@@ -23,6 +23,17 @@ SIGNATURES = {
     "htamd_prop_plan_create": [c_i64, c_i64, c_i64, c_dbl, c_dbl, c_dp, ctypes.POINTER(c_vp)],
     "htamd_prop_apply": [c_vp, c_vp, c_vp, c_i64, c_i64, c_int, c_vp],
     "htamd_prop_plan_destroy": [c_vp],
+    "htamd_shift_plan_create": [c_i64, c_i64, c_i64, c_i64, c_int, ctypes.POINTER(c_vp)],
+    "htamd_shift_plan_destroy": [c_vp],
+    "htamd_bspline_prefilter": [c_vp, c_vp, c_vp, c_i64, c_vp],
+    "htamd_bspline_interp": [c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp],
+    "htamd_bspline_interp_adj": [c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp],
+    "htamd_bspline_d1": [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp],
+    "htamd_bspline_d2": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64,
+                         c_vp, c_i64, c_vp],
+    "htamd_bspline_d1_adj_workspace_size": [c_vp, c_i64, ctypes.POINTER(c_i64)],
+    "htamd_bspline_d1_adj": [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64,
+                             c_i64, c_vp],
 }
 
 _lib = None

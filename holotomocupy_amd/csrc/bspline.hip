@@ -1,0 +1,372 @@
+// Cubic B-spline shift family for gfx950: prefilter (coeff), S, Sadj, dS,
+// d2S, dSadj.  Semantics follow src/holotomocupy/shift.py:18-221 and the
+// kernels of src/holotomocupy/cuda_kernels.py:131-843; the implementation is
+// gather-only (no atomics) so every result is deterministic.
+//
+// Sampling point of output pixel (tx, ty) of angle b (cuda_kernels.py:172-181):
+//   x = (m_b (tx - n/2) - r_b[1] + (m_b - 1)/2) + npsi/2      (same for y, r_b[0])
+// taps floor(x) + j, j = -1..2, weights phi(dx - j), phi = 6 * cubic B-spline.
+#include "htamd_bspline.hpp"
+#include "htamd_internal.hpp"
+
+namespace htamd {
+
+// ---------------------------------------------------------------------------
+// S: out[b, ty, tx] = sum_{jy,jx} wy wx c[b, iy+jy, ix+jx]   (taps in range)
+// dS:  -(dwx wy drx + wx dwy dry) c + wx wy c1
+// d2S: (d2wx wy d1x d2x + dwx dwy (d1x d2y + d1y d2x) + wx d2wy d1y d2y) c
+//      - (dwx wy d1x + wx dwy d1y) c1 - (dwx wy d2x + wx dwy d2y) c2
+// (d2S pairs Deltar1 with c1, as cuda_kernels.py:437-450 does.)
+enum { BS_S = 0, BS_D1 = 1, BS_D2 = 2 };
+
+constexpr int kTX = 64, kTY = 4;
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kTX* kTY) void bs_forward_k(BsArgs<T> a) {
+  const int64_t bid = blockIdx.x;
+  const int txt = (int)(bid % a.tiles_x);
+  const int64_t rest = bid / a.tiles_x;
+  const int tyt = (int)(rest % a.tiles_y);
+  const int64_t b = rest / a.tiles_y;
+  const int tx = txt * kTX + (int)(threadIdx.x % kTX);
+  const int ty = tyt * kTY + (int)(threadIdx.x / kTX);
+  if (tx >= a.n || ty >= a.nz) return;
+  const float mb = a.m[b * a.ms];
+  const float x = bs_coord(tx, a.n, a.npsi, mb, a.r[b * a.rs + 1]);
+  const float y = bs_coord(ty, a.nz, a.nzpsi, mb, a.r[b * a.rs + 0]);
+  const float fx = floorf(x), fy = floorf(y);
+  const int ix = (int)fx - 1, iy = (int)fy - 1;
+  float wx[4], wy[4];
+  bs_w(x - fx, wx);
+  bs_w(y - fy, wy);
+  const int64_t plane = (int64_t)a.nzpsi * a.npsi;
+  const T* c = a.c + b * plane;
+  T acc = vzero<T>();
+  if constexpr (MODE == BS_S) {
+#pragma unroll
+    for (int jy = 0; jy < 4; ++jy) {
+      const int yy = iy + jy;
+      if (yy < 0 || yy >= a.nzpsi) continue;
+      T row = vzero<T>();
+#pragma unroll
+      for (int jx = 0; jx < 4; ++jx) {
+        const int xx = ix + jx;
+        if (xx < 0 || xx >= a.npsi) continue;
+        row = vfma(wx[jx], c[(int64_t)yy * a.npsi + xx], row);
+      }
+      acc = vfma(wy[jy], row, acc);
+    }
+  } else {
+    float dwx[4], dwy[4];
+    bs_dw(x - fx, dwx);
+    bs_dw(y - fy, dwy);
+    const T* c1 = a.c1 + b * plane;
+    if constexpr (MODE == BS_D1) {
+      const float drx = a.dr1[b * a.d1s + 1], dry = a.dr1[b * a.d1s + 0];
+#pragma unroll
+      for (int jy = 0; jy < 4; ++jy) {
+        const int yy = iy + jy;
+        if (yy < 0 || yy >= a.nzpsi) continue;
+#pragma unroll
+        for (int jx = 0; jx < 4; ++jx) {
+          const int xx = ix + jx;
+          if (xx < 0 || xx >= a.npsi) continue;
+          const int64_t k = (int64_t)yy * a.npsi + xx;
+          const float w = dwx[jx] * wy[jy] * drx + wx[jx] * dwy[jy] * dry;
+          acc = vfma(-w, c[k], acc);
+          acc = vfma(wx[jx] * wy[jy], c1[k], acc);
+        }
+      }
+    } else {
+      float d2wx[4], d2wy[4];
+      bs_d2w(x - fx, d2wx);
+      bs_d2w(y - fy, d2wy);
+      const T* c2 = a.c2 + b * plane;
+      const float d1x = a.dr1[b * a.d1s + 1], d1y = a.dr1[b * a.d1s + 0];
+      const float d2x = a.dr2[b * a.d2s + 1], d2y = a.dr2[b * a.d2s + 0];
+      const float cross = d1x * d2y + d1y * d2x;
+#pragma unroll
+      for (int jy = 0; jy < 4; ++jy) {
+        const int yy = iy + jy;
+        if (yy < 0 || yy >= a.nzpsi) continue;
+#pragma unroll
+        for (int jx = 0; jx < 4; ++jx) {
+          const int xx = ix + jx;
+          if (xx < 0 || xx >= a.npsi) continue;
+          const int64_t k = (int64_t)yy * a.npsi + xx;
+          const float w = d2wx[jx] * wy[jy] * d1x * d2x + dwx[jx] * dwy[jy] * cross +
+                          wx[jx] * d2wy[jy] * d1y * d2y;
+          const float w1 = dwx[jx] * wy[jy] * d1x + wx[jx] * dwy[jy] * d1y;
+          const float w2 = dwx[jx] * wy[jy] * d2x + wx[jx] * dwy[jy] * d2y;
+          acc = vfma(w, c[k], acc);
+          acc = vfma(-w1, c1[k], acc);
+          acc = vfma(-w2, c2[k], acc);
+        }
+      }
+    }
+  }
+  a.out[(b * a.nz + ty) * (int64_t)a.n + tx] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Sadj as a gather over the coefficient grid: c[b, iy, ix] =
+//   sum_{ty: iy in taps(y(ty))} wy(ty) sum_{tx: ix in taps(x(tx))} wx(tx) g[b, ty, tx].
+// Candidate pixels are bracketed analytically with a one-pixel margin and each
+// is re-evaluated with the forward's exact float expression, so the weights
+// are bitwise those of S.
+__device__ __forceinline__ void bs_candidates(int i, int npix, int npsi, float mb, float rr, int& lo,
+                                              int& hi) {
+  const float off = bs_coord(0, npix, npsi, mb, rr) + mb * (float)(npix / 2);
+  const float a = ((float)i - 2.0f - off) / mb + (float)(npix / 2);
+  const float bnd = ((float)i + 2.0f - off) / mb + (float)(npix / 2);
+  lo = (int)floorf(fminf(a, bnd)) - 1;
+  hi = (int)ceilf(fmaxf(a, bnd)) + 1;
+  lo = lo < 0 ? 0 : lo;
+  hi = hi > npix - 1 ? npix - 1 : hi;
+}
+
+// weight with which pixel coordinate `x` reaches coefficient index i (0 if none)
+__device__ __forceinline__ float bs_reach(float x, int i) {
+  const float fx = floorf(x);
+  const int j = i - (int)fx;  // tap offset, valid in [-1, 2]
+  if (j < -1 || j > 2) return 0.0f;
+  float w[4];
+  bs_w(x - fx, w);
+  return w[j + 1];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTX* kTY) void bs_adjoint_k(BsArgs<T> a) {
+  // here a.c is the input g [B, nz, n] and a.out the coefficients [B, nzpsi, npsi]
+  const int64_t bid = blockIdx.x;
+  const int txt = (int)(bid % a.tiles_x);
+  const int64_t rest = bid / a.tiles_x;
+  const int tyt = (int)(rest % a.tiles_y);
+  const int64_t b = rest / a.tiles_y;
+  const int ix = txt * kTX + (int)(threadIdx.x % kTX);
+  const int iy = tyt * kTY + (int)(threadIdx.x / kTX);
+  if (ix >= a.npsi || iy >= a.nzpsi) return;
+  const float mb = a.m[b * a.ms];
+  const float rx = a.r[b * a.rs + 1], ry = a.r[b * a.rs + 0];
+  int xlo, xhi, ylo, yhi;
+  bs_candidates(ix, a.n, a.npsi, mb, rx, xlo, xhi);
+  bs_candidates(iy, a.nz, a.nzpsi, mb, ry, ylo, yhi);
+  const T* g = a.c + b * (int64_t)a.nz * a.n;
+  T acc = vzero<T>();
+  for (int ty = ylo; ty <= yhi; ++ty) {
+    const float wyv = bs_reach(bs_coord(ty, a.nz, a.nzpsi, mb, ry), iy);
+    if (wyv == 0.0f) continue;
+    T row = vzero<T>();
+    for (int tx = xlo; tx <= xhi; ++tx) {
+      const float wxv = bs_reach(bs_coord(tx, a.n, a.npsi, mb, rx), ix);
+      if (wxv != 0.0f) row = vfma(wxv, g[(int64_t)ty * a.n + tx], row);
+    }
+    acc = vfma(wyv, row, acc);
+  }
+  a.out[(b * a.nzpsi + iy) * (int64_t)a.npsi + ix] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// dSadj reductions: per angle, out2[b,0] = Re<g, dt1>, out2[b,1] = Re<g, dt2>
+// with dt1 = -sum wx dwy c, dt2 = -sum dwx wy c at every pixel.  Stage 1 writes
+// one partial pair per workgroup; stage 2 sums them per angle in a fixed order.
+template <typename T>
+__global__ __launch_bounds__(kTX* kTY) void bs_dadj_partial_k(BsArgs<T> a, const T* __restrict__ g,
+                                                              float2* __restrict__ partial) {
+  const int64_t bid = blockIdx.x;
+  const int txt = (int)(bid % a.tiles_x);
+  const int64_t rest = bid / a.tiles_x;
+  const int tyt = (int)(rest % a.tiles_y);
+  const int64_t b = rest / a.tiles_y;
+  const int tx = txt * kTX + (int)(threadIdx.x % kTX);
+  const int ty = tyt * kTY + (int)(threadIdx.x / kTX);
+  float s1 = 0.0f, s2 = 0.0f;
+  if (tx < a.n && ty < a.nz) {
+    const float mb = a.m[b * a.ms];
+    const float x = bs_coord(tx, a.n, a.npsi, mb, a.r[b * a.rs + 1]);
+    const float y = bs_coord(ty, a.nz, a.nzpsi, mb, a.r[b * a.rs + 0]);
+    const float fx = floorf(x), fy = floorf(y);
+    const int ix = (int)fx - 1, iy = (int)fy - 1;
+    float wx[4], wy[4], dwx[4], dwy[4];
+    bs_w(x - fx, wx);
+    bs_w(y - fy, wy);
+    bs_dw(x - fx, dwx);
+    bs_dw(y - fy, dwy);
+    const T* c = a.c + b * (int64_t)a.nzpsi * a.npsi;
+    T dt1 = vzero<T>(), dt2 = vzero<T>();
+#pragma unroll
+    for (int jy = 0; jy < 4; ++jy) {
+      const int yy = iy + jy;
+      if (yy < 0 || yy >= a.nzpsi) continue;
+#pragma unroll
+      for (int jx = 0; jx < 4; ++jx) {
+        const int xx = ix + jx;
+        if (xx < 0 || xx >= a.npsi) continue;
+        const T cv = c[(int64_t)yy * a.npsi + xx];
+        dt1 = vfma(-wx[jx] * dwy[jy], cv, dt1);
+        dt2 = vfma(-dwx[jx] * wy[jy], cv, dt2);
+      }
+    }
+    const T gv = g[(b * a.nz + ty) * (int64_t)a.n + tx];
+    s1 = vredot(gv, dt1);
+    s2 = vredot(gv, dt2);
+  }
+  const float2 tot = block_sum2<kTX * kTY>(make_float2(s1, s2));
+  if (threadIdx.x == 0) partial[bid] = tot;
+}
+
+__global__ __launch_bounds__(256) void bs_dadj_final_k(const float2* __restrict__ partial,
+                                                       int64_t per_angle, float* __restrict__ out2) {
+  const int64_t b = blockIdx.x;
+  double s1 = 0.0, s2 = 0.0;
+  for (int64_t i = threadIdx.x; i < per_angle; i += 256) {
+    const float2 p = partial[b * per_angle + i];
+    s1 += (double)p.x;
+    s2 += (double)p.y;
+  }
+  __shared__ double r1[256], r2[256];
+  r1[threadIdx.x] = s1;
+  r2[threadIdx.x] = s2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      r1[threadIdx.x] += r1[threadIdx.x + w];
+      r2[threadIdx.x] += r2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out2[2 * b + 0] = (float)r1[0];
+    out2[2 * b + 1] = (float)r2[0];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Prefilter (coeff): periodic separable FIR with the exact periodic inverse of
+// the [1 4 1] circulant (shift.py:25-30, 74-82), taps |d| <= kPfH (host-built,
+// FP64; the dropped tail is below 1e-8 relative).  One pass: a (TY+2H) x
+// (TX+2H) tile with periodic halo is staged in LDS, filtered along x into a
+// second LDS tile, then along y, and written once.
+constexpr int kPfTX = 64, kPfTY = 32;
+
+template <typename T>
+__global__ __launch_bounds__(256) void bs_prefilter_k(const T* __restrict__ in, T* __restrict__ out,
+                                                      int nzpsi, int npsi, int tiles_x, int tiles_y,
+                                                      PrefilterTaps taps) {
+  constexpr int H = kPfH;
+  constexpr int LX = kPfTX + 2 * H, LY = kPfTY + 2 * H;
+  __shared__ T s_in[LY][LX + 1];
+  __shared__ T s_tmp[LY][kPfTX + 1];
+  const int64_t bid = blockIdx.x;
+  const int txt = (int)(bid % tiles_x);
+  const int64_t rest = bid / tiles_x;
+  const int tyt = (int)(rest % tiles_y);
+  const int64_t b = rest / tiles_y;
+  const int x0 = txt * kPfTX, y0 = tyt * kPfTY;
+  const int64_t plane = (int64_t)nzpsi * npsi;
+  const T* src = in + b * plane;
+  for (int i = threadIdx.x; i < LY * LX; i += 256) {
+    const int ly = i / LX, lx = i % LX;
+    int gy = (y0 + ly - H) % nzpsi;
+    gy = gy < 0 ? gy + nzpsi : gy;
+    int gx = (x0 + lx - H) % npsi;
+    gx = gx < 0 ? gx + npsi : gx;
+    s_in[ly][lx] = src[(int64_t)gy * npsi + gx];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < LY * kPfTX; i += 256) {
+    const int ly = i / kPfTX, lx = i % kPfTX;
+    T acc = vzero<T>();
+#pragma unroll
+    for (int d = 0; d <= 2 * H; ++d) acc = vfma(taps.wx[d], s_in[ly][lx + d], acc);
+    s_tmp[ly][lx] = acc;
+  }
+  __syncthreads();
+  T* dst = out + b * plane;
+  for (int i = threadIdx.x; i < kPfTY * kPfTX; i += 256) {
+    const int ly = i / kPfTX, lx = i % kPfTX;
+    if (y0 + ly >= nzpsi || x0 + lx >= npsi) continue;
+    T acc = vzero<T>();
+#pragma unroll
+    for (int d = 0; d <= 2 * H; ++d) acc = vfma(taps.wy[d], s_tmp[ly + d][lx], acc);
+    dst[(int64_t)(y0 + ly) * npsi + x0 + lx] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+template <typename T>
+static void tiles_of(BsArgs<T>& a, bool coeff_grid) {
+  const int w = coeff_grid ? a.npsi : a.n, h = coeff_grid ? a.nzpsi : a.nz;
+  a.tiles_x = (w + kTX - 1) / kTX;
+  a.tiles_y = (h + kTY - 1) / kTY;
+}
+
+template <typename T>
+hipError_t bs_launch_forward(BsArgs<T> a, int mode, hipStream_t st) {
+  tiles_of(a, false);
+  const int64_t grid = a.batch * (int64_t)a.tiles_x * a.tiles_y;
+  if (grid == 0) return hipSuccess;
+  if (mode == BS_S)
+    hipLaunchKernelGGL((bs_forward_k<T, BS_S>), dim3((unsigned)grid), dim3(kTX * kTY), 0, st, a);
+  else if (mode == BS_D1)
+    hipLaunchKernelGGL((bs_forward_k<T, BS_D1>), dim3((unsigned)grid), dim3(kTX * kTY), 0, st, a);
+  else
+    hipLaunchKernelGGL((bs_forward_k<T, BS_D2>), dim3((unsigned)grid), dim3(kTX * kTY), 0, st, a);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t bs_launch_adjoint(BsArgs<T> a, hipStream_t st) {
+  tiles_of(a, true);
+  const int64_t grid = a.batch * (int64_t)a.tiles_x * a.tiles_y;
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL((bs_adjoint_k<T>), dim3((unsigned)grid), dim3(kTX * kTY), 0, st, a);
+  return hipGetLastError();
+}
+
+template <typename T>
+int64_t bs_dadj_partials(const BsArgs<T>& a0) {
+  BsArgs<T> a = a0;
+  tiles_of(a, false);
+  return a.batch * (int64_t)a.tiles_x * a.tiles_y;
+}
+
+template <typename T>
+hipError_t bs_launch_dadj_reduce(BsArgs<T> a, const T* g, float2* partial, float* out2,
+                                 hipStream_t st) {
+  tiles_of(a, false);
+  const int64_t per_angle = (int64_t)a.tiles_x * a.tiles_y;
+  const int64_t grid = a.batch * per_angle;
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL((bs_dadj_partial_k<T>), dim3((unsigned)grid), dim3(kTX * kTY), 0, st, a, g,
+                     partial);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(bs_dadj_final_k, dim3((unsigned)a.batch), dim3(256), 0, st, partial, per_angle,
+                     out2);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t bs_launch_prefilter(const T* in, T* out, int64_t batch, int nzpsi, int npsi,
+                               const PrefilterTaps& taps, hipStream_t st) {
+  const int tx = (npsi + kPfTX - 1) / kPfTX, ty = (nzpsi + kPfTY - 1) / kPfTY;
+  const int64_t grid = batch * (int64_t)tx * ty;
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL((bs_prefilter_k<T>), dim3((unsigned)grid), dim3(256), 0, st, in, out, nzpsi,
+                     npsi, tx, ty, taps);
+  return hipGetLastError();
+}
+
+#define HTAMD_BS_INST(T)                                                                         \
+  template hipError_t bs_launch_forward<T>(BsArgs<T>, int, hipStream_t);                         \
+  template hipError_t bs_launch_adjoint<T>(BsArgs<T>, hipStream_t);                              \
+  template int64_t bs_dadj_partials<T>(const BsArgs<T>&);                                        \
+  template hipError_t bs_launch_dadj_reduce<T>(BsArgs<T>, const T*, float2*, float*, hipStream_t); \
+  template hipError_t bs_launch_prefilter<T>(const T*, T*, int64_t, int, int, const PrefilterTaps&, \
+                                             hipStream_t);
+HTAMD_BS_INST(float)
+HTAMD_BS_INST(float2)
+
+}  // namespace htamd

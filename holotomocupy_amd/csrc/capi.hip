@@ -19,12 +19,12 @@ static thread_local std::string g_last_error;
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 
-static int fail(int code, const std::string& msg) {
+int fail(int code, const std::string& msg) {
   set_error(msg);
   return code;
 }
 
-static int hip_fail(hipError_t e, const char* where) {
+int hip_fail(hipError_t e, const char* where) {
   return fail(HTAMD_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
 }
 

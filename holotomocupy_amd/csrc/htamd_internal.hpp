@@ -7,6 +7,9 @@
 namespace htamd {
 
 void set_error(const std::string& msg);
+// set the message and return `code` / HTAMD_ERR_HIP
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* where);
 
 // ---- Fresnel propagator (fresnel.hip) -------------------------------------
 // Smallest / largest power-of-two transform length handled by the LDS FFT.
