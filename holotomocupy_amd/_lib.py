@@ -23,6 +23,13 @@ SIGNATURES = {
     "htamd_prop_plan_create": [c_i64, c_i64, c_i64, c_dbl, c_dbl, c_dp, ctypes.POINTER(c_vp)],
     "htamd_prop_apply": [c_vp, c_vp, c_vp, c_i64, c_i64, c_int, c_vp],
     "htamd_prop_plan_destroy": [c_vp],
+    "htamd_prop_apply_strided": [c_vp, c_vp, c_i64, c_vp, c_i64, c_i64, c_i64, c_int, c_vp],
+    "htamd_cas_probe_combo": [c_vp] + [c_vp] * 8 + [c_i64, c_i64, c_i64, ctypes.c_float, c_vp],
+    "htamd_cas_expi": [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_vp],
+    "htamd_cas_residual": [c_vp, c_vp, c_vp, c_i64, ctypes.c_float, c_vp],
+    "htamd_cas_probe_grad": [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_int, c_vp],
+    "htamd_cas_reduce_workspace_size": [c_i64, ctypes.POINTER(c_i64)],
+    "htamd_cas_reduce": [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp],
     "htamd_shift_plan_create": [c_i64, c_i64, c_i64, c_i64, c_int, ctypes.POINTER(c_vp)],
     "htamd_shift_plan_destroy": [c_vp],
     "htamd_bspline_prefilter": [c_vp, c_vp, c_vp, c_i64, c_vp],

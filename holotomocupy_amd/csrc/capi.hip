@@ -141,14 +141,17 @@ int htamd_prop_plan_destroy(htamd_prop_plan* p) {
   return HTAMD_OK;
 }
 
-int htamd_prop_apply(const htamd_prop_plan* p, const void* in, void* out, int64_t batch, int64_t j,
-                     int adjoint, void* stream) {
+int htamd_prop_apply_strided(const htamd_prop_plan* p, const void* in, int64_t in_stride, void* out,
+                             int64_t out_stride, int64_t batch, int64_t j, int adjoint, void* stream) {
   if (!p) return fail(HTAMD_ERR_ARG, "prop_apply: plan is NULL");
   if (batch < 0) return fail(HTAMD_ERR_ARG, "prop_apply: negative batch");
   if (j < 0 || j >= p->ndist)
     return fail(HTAMD_ERR_ARG, "prop_apply: distance index " + std::to_string(j) + " out of range");
   if (batch == 0) return HTAMD_OK;
   if (!in || !out) return fail(HTAMD_ERR_ARG, "prop_apply: NULL buffer");
+  const int64_t field = p->n * p->nz;
+  if (in_stride < field || out_stride < field)
+    return fail(HTAMD_ERR_ARG, "prop_apply: field strides must be >= nz*n");
   int dev = -1;
   (void)hipGetDevice(&dev);
   if (dev != p->device)
@@ -158,12 +161,18 @@ int htamd_prop_apply(const htamd_prop_plan* p, const void* in, void* out, int64_
   const float2* tabx = p->tab_x + (j * 2 + a) * 3 * (p->n / 2);
   const float2* taby = p->tab_y + (j * 2 + a) * 3 * (p->nz / 2);
   hipStream_t st = (hipStream_t)stream;
-  hipError_t e = launch_fresnel_rows(p->n, (const float2*)in, (float2*)out, batch * p->nz, p->tw_x,
-                                     tabx, st);
+  hipError_t e = launch_fresnel_rows(p->n, p->nz, (const float2*)in, in_stride, (float2*)out,
+                                     out_stride, batch, p->tw_x, tabx, st);
   if (e != hipSuccess) return hip_fail(e, "prop_apply: row pass");
-  e = launch_fresnel_cols(p->nz, p->n, (float2*)out, batch, p->tw_y, taby, st);
+  e = launch_fresnel_cols(p->nz, p->n, (float2*)out, out_stride, batch, p->tw_y, taby, st);
   if (e != hipSuccess) return hip_fail(e, "prop_apply: column pass");
   return HTAMD_OK;
+}
+
+int htamd_prop_apply(const htamd_prop_plan* p, const void* in, void* out, int64_t batch, int64_t j,
+                     int adjoint, void* stream) {
+  const int64_t field = p ? p->n * p->nz : 0;
+  return htamd_prop_apply_strided(p, in, field, out, field, batch, j, adjoint, stream);
 }
 
 }  // extern "C"

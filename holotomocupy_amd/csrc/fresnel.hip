@@ -54,8 +54,9 @@ constexpr int kRowThreads = 256;
 
 // Rows of length N, ROWS = 256/(N/E) rows per workgroup.
 template <int N, int E>
-__global__ __launch_bounds__(kRowThreads) void fresnel_rows_k(const float2* in, float2* out,
-                                                              int64_t nrows,
+__global__ __launch_bounds__(kRowThreads) void fresnel_rows_k(const float2* in, int64_t in_stride,
+                                                              float2* out, int64_t out_stride,
+                                                              int nz, int64_t nrows,
                                                               const float2* __restrict__ tw,
                                                               const float2* __restrict__ tab) {
   constexpr int T = N / E;
@@ -64,8 +65,9 @@ __global__ __launch_bounds__(kRowThreads) void fresnel_rows_k(const float2* in, 
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   const int64_t row = (int64_t)blockIdx.x * ROWS + r;
   const bool valid = row < nrows;
+  const int64_t fb = valid ? row / nz : 0, fy = valid ? row % nz : 0;
   float2* s = lds + r * N;
-  const float2* src = in + row * N;
+  const float2* src = in + fb * in_stride + fy * N;
   FftTw<N, E> twr;
   fft_load_twiddles<N, E>(twr, t, tw);
 #pragma unroll
@@ -76,7 +78,7 @@ __global__ __launch_bounds__(kRowThreads) void fresnel_rows_k(const float2* in, 
   __syncthreads();
   fresnel_1d<N, E>(s, t, twr, tab);
   if (valid) {
-    float2* dst = out + row * N;
+    float2* dst = out + fb * out_stride + fy * N;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = t + e * T;
@@ -89,7 +91,7 @@ __global__ __launch_bounds__(kRowThreads) void fresnel_rows_k(const float2* in, 
 // workgroup, in place.  Workgroups are mapped XCD-contiguously so neighbouring
 // column tiles (which share cache lines) run under the same L2.
 template <int N, int E, int C>
-__global__ __launch_bounds__(C*(N / E)) void fresnel_cols_k(float2* data, int n,
+__global__ __launch_bounds__(C*(N / E)) void fresnel_cols_k(float2* data, int64_t stride, int n,
                                                             int tiles_per_field,
                                                             const float2* __restrict__ tw,
                                                             const float2* __restrict__ tab) {
@@ -100,7 +102,7 @@ __global__ __launch_bounds__(C*(N / E)) void fresnel_cols_k(float2* data, int n,
   const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const int64_t b = tile / tiles_per_field;
   const int x0 = (int)(tile % tiles_per_field) * C;
-  float2* base = data + b * (int64_t)N * n + x0;
+  float2* base = data + b * stride + x0;
   const int cc = threadIdx.x / T, t = threadIdx.x % T;
   FftTw<N, E> twr;
   fft_load_twiddles<N, E>(twr, t, tw);
@@ -125,76 +127,73 @@ bool fft_size_supported(int64_t n) {
 }
 
 template <int N>
-static hipError_t rows_n(const float2* in, float2* out, int64_t nrows, const float2* tw,
-                         const float2* tab, hipStream_t st) {
+static hipError_t rows_n(const float2* in, int64_t in_stride, float2* out, int64_t out_stride,
+                         int64_t nz, int64_t batch, const float2* tw, const float2* tab,
+                         hipStream_t st) {
   constexpr int E = 8;
   constexpr int ROWS = kRowThreads / (N / E);
+  const int64_t nrows = batch * nz;
   const int64_t grid = (nrows + ROWS - 1) / ROWS;
   if (grid == 0) return hipSuccess;
   hipLaunchKernelGGL((fresnel_rows_k<N, E>), dim3((unsigned)grid), dim3(kRowThreads), 0, st, in,
-                     out, nrows, tw, tab);
+                     in_stride, out, out_stride, (int)nz, nrows, tw, tab);
   return hipGetLastError();
 }
 
-hipError_t launch_fresnel_rows(int64_t n, const float2* in, float2* out, int64_t nrows,
-                               const float2* tw, const float2* tab, hipStream_t st) {
+hipError_t launch_fresnel_rows(int64_t n, int64_t nz, const float2* in, int64_t in_stride,
+                               float2* out, int64_t out_stride, int64_t batch, const float2* tw,
+                               const float2* tab, hipStream_t st) {
+#define HTAMD_ROWS_CASE(NN) \
+  case NN: return rows_n<NN>(in, in_stride, out, out_stride, nz, batch, tw, tab, st);
   switch (n) {
-    case 8: return rows_n<8>(in, out, nrows, tw, tab, st);
-    case 16: return rows_n<16>(in, out, nrows, tw, tab, st);
-    case 32: return rows_n<32>(in, out, nrows, tw, tab, st);
-    case 64: return rows_n<64>(in, out, nrows, tw, tab, st);
-    case 128: return rows_n<128>(in, out, nrows, tw, tab, st);
-    case 256: return rows_n<256>(in, out, nrows, tw, tab, st);
-    case 512: return rows_n<512>(in, out, nrows, tw, tab, st);
-    case 1024: return rows_n<1024>(in, out, nrows, tw, tab, st);
-    case 2048: return rows_n<2048>(in, out, nrows, tw, tab, st);
+    HTAMD_ROWS_CASE(8) HTAMD_ROWS_CASE(16) HTAMD_ROWS_CASE(32) HTAMD_ROWS_CASE(64)
+    HTAMD_ROWS_CASE(128) HTAMD_ROWS_CASE(256) HTAMD_ROWS_CASE(512) HTAMD_ROWS_CASE(1024)
+    HTAMD_ROWS_CASE(2048)
     default: return hipErrorInvalidValue;
   }
+#undef HTAMD_ROWS_CASE
 }
 
 template <int N, int C>
-static hipError_t cols_nc(int64_t n, float2* data, int64_t batch, const float2* tw,
+static hipError_t cols_nc(int64_t n, float2* data, int64_t stride, int64_t batch, const float2* tw,
                           const float2* tab, hipStream_t st) {
   constexpr int E = 8;
   const int64_t tiles_per_field = n / C;
   const int64_t grid = batch * tiles_per_field;
   if (grid == 0) return hipSuccess;
   hipLaunchKernelGGL((fresnel_cols_k<N, E, C>), dim3((unsigned)grid), dim3(C * (N / E)), 0, st,
-                     data, (int)n, (int)tiles_per_field, tw, tab);
+                     data, stride, (int)n, (int)tiles_per_field, tw, tab);
   return hipGetLastError();
 }
 
 // Columns per workgroup: as many as fit 1024 threads and ~64 KB of LDS, and
 // divide the row length n (n is even, so C = 2 always divides).
 template <int N>
-static hipError_t cols_n(int64_t n, float2* data, int64_t batch, const float2* tw,
+static hipError_t cols_n(int64_t n, float2* data, int64_t stride, int64_t batch, const float2* tw,
                          const float2* tab, hipStream_t st) {
   constexpr int T = N / 8;
   constexpr int CT = 1024 / T;
   constexpr int CL = 8192 / N;
   constexpr int CMAX0 = CT < CL ? CT : CL;
   constexpr int CMAX = CMAX0 > 16 ? 16 : CMAX0;
-  if (CMAX >= 16 && n % 16 == 0) return cols_nc<N, (CMAX >= 16 ? 16 : 1)>(n, data, batch, tw, tab, st);
-  if (CMAX >= 8 && n % 8 == 0) return cols_nc<N, (CMAX >= 8 ? 8 : 1)>(n, data, batch, tw, tab, st);
-  if (CMAX >= 4 && n % 4 == 0) return cols_nc<N, (CMAX >= 4 ? 4 : 1)>(n, data, batch, tw, tab, st);
-  if (n % 2 == 0) return cols_nc<N, 2>(n, data, batch, tw, tab, st);
-  return cols_nc<N, 1>(n, data, batch, tw, tab, st);
+  if (CMAX >= 16 && n % 16 == 0) return cols_nc<N, (CMAX >= 16 ? 16 : 1)>(n, data, stride, batch, tw, tab, st);
+  if (CMAX >= 8 && n % 8 == 0) return cols_nc<N, (CMAX >= 8 ? 8 : 1)>(n, data, stride, batch, tw, tab, st);
+  if (CMAX >= 4 && n % 4 == 0) return cols_nc<N, (CMAX >= 4 ? 4 : 1)>(n, data, stride, batch, tw, tab, st);
+  if (n % 2 == 0) return cols_nc<N, 2>(n, data, stride, batch, tw, tab, st);
+  return cols_nc<N, 1>(n, data, stride, batch, tw, tab, st);
 }
 
-hipError_t launch_fresnel_cols(int64_t nz, int64_t n, float2* data, int64_t batch,
+hipError_t launch_fresnel_cols(int64_t nz, int64_t n, float2* data, int64_t stride, int64_t batch,
                                const float2* tw, const float2* tab, hipStream_t st) {
+#define HTAMD_COLS_CASE(NN) \
+  case NN: return cols_n<NN>(n, data, stride, batch, tw, tab, st);
   switch (nz) {
-    case 8: return cols_n<8>(n, data, batch, tw, tab, st);
-    case 16: return cols_n<16>(n, data, batch, tw, tab, st);
-    case 32: return cols_n<32>(n, data, batch, tw, tab, st);
-    case 64: return cols_n<64>(n, data, batch, tw, tab, st);
-    case 128: return cols_n<128>(n, data, batch, tw, tab, st);
-    case 256: return cols_n<256>(n, data, batch, tw, tab, st);
-    case 512: return cols_n<512>(n, data, batch, tw, tab, st);
-    case 1024: return cols_n<1024>(n, data, batch, tw, tab, st);
-    case 2048: return cols_n<2048>(n, data, batch, tw, tab, st);
+    HTAMD_COLS_CASE(8) HTAMD_COLS_CASE(16) HTAMD_COLS_CASE(32) HTAMD_COLS_CASE(64)
+    HTAMD_COLS_CASE(128) HTAMD_COLS_CASE(256) HTAMD_COLS_CASE(512) HTAMD_COLS_CASE(1024)
+    HTAMD_COLS_CASE(2048)
     default: return hipErrorInvalidValue;
   }
+#undef HTAMD_COLS_CASE
 }
 
 }  // namespace htamd

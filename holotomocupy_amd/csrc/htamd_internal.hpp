@@ -17,12 +17,14 @@ constexpr int64_t kFftMin = 8;
 constexpr int64_t kFftMax = 2048;
 bool fft_size_supported(int64_t n);
 
-// Row pass: nrows rows of length n, out = D_x(in) row by row (in may equal out).
+// Row pass over `batch` fields of nz rows of length n; field b starts at
+// in + b*in_stride / out + b*out_stride (elements); in may equal out.
 // tab: pair table (3*(n/2) float2) of the 1-D operator, tw: twiddles (n float2).
-hipError_t launch_fresnel_rows(int64_t n, const float2* in, float2* out, int64_t nrows,
-                               const float2* tw, const float2* tab, hipStream_t st);
-// Column pass, in place: data [batch, nz, n], columns of length nz.
-hipError_t launch_fresnel_cols(int64_t nz, int64_t n, float2* data, int64_t batch,
+hipError_t launch_fresnel_rows(int64_t n, int64_t nz, const float2* in, int64_t in_stride,
+                               float2* out, int64_t out_stride, int64_t batch, const float2* tw,
+                               const float2* tab, hipStream_t st);
+// Column pass, in place: `batch` fields [nz, n] at data + b*stride.
+hipError_t launch_fresnel_cols(int64_t nz, int64_t n, float2* data, int64_t stride, int64_t batch,
                                const float2* tw, const float2* tab, hipStream_t st);
 
 }  // namespace htamd

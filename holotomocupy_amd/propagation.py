@@ -77,3 +77,18 @@ class Propagation:
     def DT(self, big_psi, j):
         """Adjoint propagator  (propagation.py:91-112)."""
         return self._apply(big_psi, j, True)
+
+    def apply_into(self, src, dst, j, adjoint=False):
+        """dst[b] = D_j(src[b]) (or DT_j) for [B, nz, n] views whose last two
+        dimensions are contiguous (e.g. x[:, j] of a [B, ndist, nz, n] array);
+        no copies are made.  src may be dst."""
+        for t in (src, dst):
+            if (t.dtype != torch.complex64 or t.ndim != 3 or tuple(t.shape[1:]) != (self.nz, self.n)
+                    or t.stride(2) != 1 or t.stride(1) != self.n or t.device != self.device):
+                raise ValueError("apply_into needs complex64 [B, nz, n] views with contiguous rows")
+        if src.shape[0] != dst.shape[0]:
+            raise ValueError("batch mismatch")
+        j = int(j) + (self.ndist if int(j) < 0 else 0)
+        _lib.call("htamd_prop_apply_strided", self._plan, _lib.ptr(src), src.stride(0), _lib.ptr(dst),
+                  dst.stride(0), src.shape[0], j, 1 if adjoint else 0, _lib.stream_ptr())
+        return dst

@@ -55,6 +55,14 @@ int htamd_prop_plan_create(int64_t n, int64_t nz, int64_t ndist, double waveleng
 int htamd_prop_apply(const htamd_prop_plan* plan, const void* in, void* out, int64_t batch,
                      int64_t j, int adjoint, void* stream);
 
+/* As htamd_prop_apply with field b read at in + b*in_stride and written at
+ * out + b*out_stride (strides in complex elements, >= nz*n), e.g. the slice
+ * x[:, j] of a [batch, ndist, nz, n] array (stride ndist*nz*n).  Replaces the
+ * per-distance D / DT calls on array slices of the reference cascade
+ * (src/holotomocupy/rec_mpi.py:578-580, 593-598, 638-639). */
+int htamd_prop_apply_strided(const htamd_prop_plan* plan, const void* in, int64_t in_stride, void* out,
+                             int64_t out_stride, int64_t batch, int64_t j, int adjoint, void* stream);
+
 /* Replaces conv2d_destroy (src/holotomocupy/cuda/conv2d.cu:157-159). */
 int htamd_prop_plan_destroy(htamd_prop_plan* plan);
 
@@ -115,6 +123,48 @@ int htamd_bspline_d1_adj_workspace_size(const htamd_shift_plan* plan, int64_t ba
 int htamd_bspline_d1_adj(const htamd_shift_plan* plan, const void* c, const void* g, const float* r,
                          int64_t r_stride, const float* m, int64_t m_stride, void* out1, float* out2,
                          void* workspace, int64_t workspace_bytes, int64_t batch, void* stream);
+
+/* ---- pointwise cascade stages and reductions -------------------------------
+ * Complex arrays are complex64; per-angle fields are [batch, ndist, npix]
+ * (npix = nz*n), probe-like arrays [ndist, npix]. */
+
+/* out[b,j,i] = scale * sum_t p_t[j,i] q_t[b,j,i] over the non-NULL pairs.
+ * Replaces the probe products of F1 / dF1 / d2F_dF1
+ * (src/holotomocupy/rec_mpi.py:572-623). */
+int htamd_cas_probe_combo(void* out, const void* p0, const void* q0, const void* p1, const void* q1,
+                          const void* p2, const void* q2, const void* p3, const void* q3,
+                          int64_t batch, int64_t ndist, int64_t npix, float scale, void* stream);
+
+/* Pointwise exp stages of F2 (rec_mpi.py:645-724: _F2_fused, _dF2_fused,
+ * _d2F_dF2_fused, _gF2_fused) on `count` elements; x22 / y / z / w are
+ * complex64 if x_complex else float32 (in mode 3 y is the complex y12):
+ *   mode 0: out1 = exp(i x22)
+ *   mode 1: out1 = exp(i x22), out2 = out1 * i * y        (out2 may be NULL)
+ *   mode 2: out1 = exp(i x22) * (-y z) [+ exp(i x22) * i * w if w != NULL]
+ *   mode 3: out1 = -i * y * conj(exp(i x22)); real_out keeps the real part
+ *           (float32 output) as for a float32 object. */
+int htamd_cas_expi(int mode, const void* x22, const void* y, const void* z, const void* w, void* out1,
+                   void* out2, int64_t count, int x_complex, int real_out, void* stream);
+
+/* out = scale * (x - d x/|x|): the data-term gradient _gF0_fused
+ * (rec_mpi.py:555-569); d is float32. */
+int htamd_cas_residual(const void* x, const float* d, void* out, int64_t count, float scale,
+                       void* stream);
+
+/* Tail of gF1 (rec_mpi.py:626-643): y11[j] (+)= sum_b y12[b,j] conj(x12[b,j]),
+ * then y12[b,j] *= conj(x11[j]) in place; accumulate != 0 adds to y11. */
+int htamd_cas_probe_grad(void* y12, const void* x12, const void* x11, void* y11, int64_t batch,
+                         int64_t ndist, int64_t npix, int accumulate, void* stream);
+
+/* Deterministic FP64 reductions over `count` complex elements into result[0]
+ * (device double): kind 0: sum (|x|-d)^2 (F0, rec_mpi.py:517-527);
+ * kind 1: sum Re((x - d x/|x|) conj y) (dF0, rec_mpi.py:529-537);
+ * kind 2: sum (1-d/|x|) Re(y conj z) + (d/|x|) Re(l conj y) Re(l conj z)
+ *         [+ Re((x - d l) conj w)], l = x/|x| (d2F_dF0, rec_mpi.py:539-553). */
+int htamd_cas_reduce_workspace_size(int64_t count, int64_t* bytes);
+int htamd_cas_reduce(int kind, const void* x, const float* d, const void* y, const void* z,
+                     const void* w, int64_t count, double* result, void* workspace,
+                     int64_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }
