@@ -1,0 +1,189 @@
+"""BH gradient evaluations/s at a 2048x2048 detector with 4 distances on one MI355X.
+
+One step = one BH gradient evaluation (rec_mpi.Rec.gradients_cascade: object,
+probe and position gradients of the data term) for a request of A angles,
+detector 2048x2048, 4 distances, object-plane fields 1024x1024 per angle,
+inputs resident in HBM.  `value` counts gradient evaluations per second, one
+evaluation = one angle across all distances.
+
+Paths: "plain" runs the reference-structured cascade (one kernel per operator
+and distance, F3 recomputed per stage, intermediates materialised);
+"served" runs the fused multi-pass gradient behind the request loop when
+available.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--angles", type=int, default=16, help="angles per request (A)")
+    p.add_argument("--n", type=int, default=2048)
+    p.add_argument("--nobj", type=int, default=1024)
+    p.add_argument("--ndist", type=int, default=4)
+    p.add_argument("--path", choices=["auto", "plain", "served"], default="auto")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    return p.parse_args()
+
+
+def geometry(n, ndist):
+    """Physics of the reference's synthetic test (tests/test.py:14-19) at bin 0."""
+    import numpy as np
+    z1 = np.array([5.110, 5.464, 6.879, 9.817])[:ndist] * 1e-3
+    return dict(energy=17.1, detector_pixelsize=1.4760147601476e-6 * 2 * (2048 // n),
+                focustodetectordistance=1.217, z1=z1)
+
+
+def make_inputs(a, seed=1004):
+    import numpy as np
+    import torch
+    from holotomocupy_amd import phantom
+    from holotomocupy_amd.rec import Rec
+    from types import SimpleNamespace
+    rng = np.random.default_rng(seed)
+    g = geometry(a.n, a.ndist)
+    args = SimpleNamespace(ndist=a.ndist, ntheta=a.angles, nz=a.n, n=a.n, nzobj=a.nobj, nobj=a.nobj,
+                           obj_dtype="complex64", rho=[1.0, 0.05, 0.02], **g)
+    rec = Rec(args)
+    prb = phantom.probe(rng, a.ndist, a.n, a.n)
+    proj = phantom.projections(rng, a.angles, a.nobj, a.nobj)
+    pos = phantom.positions(rng, a.angles, a.ndist)
+    dev = lambda x: torch.from_numpy(x).cuda()  # noqa: E731
+    truth = {"prb": dev(prb), "proj": dev(proj), "pos": dev(pos)}
+    with torch.no_grad():
+        amp = rec.forward(truth).abs()
+        data = (amp * (1 + 0.01 * torch.randn_like(amp))).float().contiguous()
+    vars_ = {"prb": truth["prb"] * 0.9, "proj": truth["proj"] * 0.8,
+             "pos": truth["pos"] + 0.3 * torch.randn_like(truth["pos"])}
+    return rec, vars_, data, (prb, proj, pos, args, g)
+
+
+def plain_step(rec, vars_, grads, data):
+    grads["prb"].zero_()
+    rec.gradients_cascade(vars_, grads, data)
+
+
+def time_steps(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for s, e in evs:
+        s.record()
+        fn()
+        e.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = sorted(s.elapsed_time(e) for s, e in evs)
+    return wall, per
+
+
+def roofline_fresnel_cols(rec, a, reps=20):
+    """Dominant kernel of the plain path: the Fresnel column pass on a batch of
+    A detector fields.  Algorithmic bytes = read + write of each field
+    (2 * 8 * nz * n per field).  Timed live with events on the launch stream."""
+    import torch
+    from holotomocupy_amd import _lib
+    B = a.angles
+    buf = torch.randn(B, a.n, a.n, dtype=torch.complex64, device="cuda")
+    st = torch.cuda.current_stream()
+    args = (rec.cl_prop._plan, _lib.ptr(buf), a.n * a.n, _lib.ptr(buf), a.n * a.n, B, 0, 0, 1,
+            _lib.stream_ptr(st))
+    for _ in range(3):
+        _lib.call("htamd_prop_pass", *args)
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record(st)
+    for _ in range(reps):
+        _lib.call("htamd_prop_pass", *args)
+    e.record(st)
+    torch.cuda.synchronize()
+    us = s.elapsed_time(e) * 1e3 / reps
+    nbytes = 2 * 8 * a.n * a.n * B
+    achieved = nbytes / (us * 1e-6) / 1e9
+    del buf
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "fresnel_cols_k (Fresnel column pass, %d fields of %dx%d)" % (B, a.n, a.n),
+            "bytes_per_launch": nbytes, "avg_us": round(us, 2)}
+
+
+def cpu_baseline(a, host):
+    """Oracle (NumPy/SciPy port of the reference cascade) on one angle of the
+    same workload: one BH gradient evaluation, timed once."""
+    import numpy as np
+    from oracle import cascade as oc
+    from oracle import propagation as op
+    from oracle import shift as osh
+    from holotomocupy_amd.rec import physics
+    prb, proj, pos, args, g = host
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cores = max(1, min(cores, os.cpu_count() or 1))
+    op.WORKERS = cores
+    osh.WORKERS = cores
+    ph = physics(g["energy"], g["z1"], g["focustodetectordistance"], g["detector_pixelsize"])
+    prop = op.Propagation(a.n, a.n, 1, a.ndist, ph.wavelength, ph.voxelsize, ph.distance)
+    sh = osh.Shift(a.n, a.nobj, a.n, a.nobj)
+    mag = (1 / ph.norm_magnifications)[None, :]
+    cas = oc.Cascade(prop, sh, a.ndist, a.n, a.n, a.nobj, a.nobj, a.angles * a.ndist * a.n * a.n, mag)
+    x = [prb.astype(np.complex128), proj[:1].astype(np.complex128), pos[:1].astype(np.float64)]
+    d = np.abs(cas.forward(*x)).astype(np.float64)
+    t0 = time.perf_counter()
+    cas.gradients(*x, d)
+    dt = time.perf_counter() - t0
+    return {"value": round(1.0 / dt, 5), "unit": "evals/s", "cores": cores, "kind": "port",
+            "sample": f"1 angle x {a.ndist} distances at {a.n}^2 / object {a.nobj}^2, FP64 NumPy/SciPy "
+                      f"oracle cascade (scipy.fft workers={cores}), one run of {dt:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {a.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    torch.cuda.set_device(0)
+    rec, vars_, data, host = make_inputs(a)
+    grads = {"prb": torch.zeros_like(vars_["prb"]), "proj": torch.empty_like(vars_["proj"]),
+             "pos": torch.empty_like(vars_["pos"])}
+    path = "plain" if a.path in ("auto", "plain") else "served"
+    if path != "plain":
+        raise SystemExit("served path not built yet")
+    fn = lambda: plain_step(rec, vars_, grads, data)  # noqa: E731
+    wall, per = time_steps(fn, a.steps, a.warmup)
+    ms = wall * 1e3 / a.steps
+    value = a.angles * a.steps / wall
+    roof = roofline_fresnel_cols(rec, a)
+    out = {
+        "metric": "BH gradient evals/s at 2048^2 (served vs plain), % HBM roofline, p50/p99 latency",
+        "value": round(value, 3), "unit": "evals/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64", "data": "synthetic (seeded phantom projections, Gaussian probe, noisy amplitudes)",
+        "config": {"workload": f"BH gradient (gradients_cascade), detector {a.n}x{a.n}, {a.ndist} distances, "
+                               f"object plane {a.nobj}x{a.nobj}, {a.angles} angles per request",
+                   "path": path, "angles_per_step": a.angles, "n": a.n, "ndist": a.ndist, "nobj": a.nobj},
+        "latency_ms": {"p50": round(per[len(per) // 2], 3), "p99": round(per[min(len(per) - 1, int(0.99 * len(per)))], 3)},
+        "plain_value": round(value, 3),
+        "roofline": roof,
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(a, host)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

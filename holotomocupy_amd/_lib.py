@@ -24,6 +24,7 @@ SIGNATURES = {
     "htamd_prop_apply": [c_vp, c_vp, c_vp, c_i64, c_i64, c_int, c_vp],
     "htamd_prop_plan_destroy": [c_vp],
     "htamd_prop_apply_strided": [c_vp, c_vp, c_i64, c_vp, c_i64, c_i64, c_i64, c_int, c_vp],
+    "htamd_prop_pass": [c_vp, c_vp, c_i64, c_vp, c_i64, c_i64, c_i64, c_int, c_int, c_vp],
     "htamd_cas_probe_combo": [c_vp] + [c_vp] * 8 + [c_i64, c_i64, c_i64, ctypes.c_float, c_vp],
     "htamd_cas_expi": [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_vp],
     "htamd_cas_residual": [c_vp, c_vp, c_vp, c_i64, ctypes.c_float, c_vp],

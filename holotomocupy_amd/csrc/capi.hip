@@ -169,6 +169,29 @@ int htamd_prop_apply_strided(const htamd_prop_plan* p, const void* in, int64_t i
   return HTAMD_OK;
 }
 
+int htamd_prop_pass(const htamd_prop_plan* p, const void* in, int64_t in_stride, void* out,
+                    int64_t out_stride, int64_t batch, int64_t j, int adjoint, int axis, void* stream) {
+  if (!p) return fail(HTAMD_ERR_ARG, "prop_pass: plan is NULL");
+  if (batch < 0 || j < 0 || j >= p->ndist || (axis != 0 && axis != 1))
+    return fail(HTAMD_ERR_ARG, "prop_pass: bad batch, distance or axis");
+  if (batch == 0) return HTAMD_OK;
+  if (!in || !out) return fail(HTAMD_ERR_ARG, "prop_pass: NULL buffer");
+  const int64_t field = p->n * p->nz;
+  if (in_stride < field || out_stride < field)
+    return fail(HTAMD_ERR_ARG, "prop_pass: field strides must be >= nz*n");
+  if (axis == 1 && in != out) return fail(HTAMD_ERR_ARG, "prop_pass: the column pass is in place");
+  const int a = adjoint ? 1 : 0;
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e;
+  if (axis == 0)
+    e = launch_fresnel_rows(p->n, p->nz, (const float2*)in, in_stride, (float2*)out, out_stride, batch,
+                            p->tw_x, p->tab_x + (j * 2 + a) * 3 * (p->n / 2), st);
+  else
+    e = launch_fresnel_cols(p->nz, p->n, (float2*)out, out_stride, batch, p->tw_y,
+                            p->tab_y + (j * 2 + a) * 3 * (p->nz / 2), st);
+  return e == hipSuccess ? HTAMD_OK : hip_fail(e, "prop_pass");
+}
+
 int htamd_prop_apply(const htamd_prop_plan* p, const void* in, void* out, int64_t batch, int64_t j,
                      int adjoint, void* stream) {
   const int64_t field = p ? p->n * p->nz : 0;

@@ -66,9 +66,13 @@ __device__ __forceinline__ void dft(float2* a) {
 }
 
 // ---- compile-time stage schedule -------------------------------------------
+// Stage radix: at most 8 (in-register DFTs up to radix 8); a thread owning
+// E > 8 values performs E/R butterflies per stage.
+constexpr int kMaxRadix = 8;
 template <int N, int E, int NS>
 constexpr int stage_radix() {
-  return (N / NS) >= E ? E : (N / NS);
+  constexpr int R0 = E < kMaxRadix ? E : kMaxRadix;
+  return (N / NS) >= R0 ? R0 : (N / NS);
 }
 // number of base twiddles a thread holds for stages starting at span NS
 template <int N, int E, int NS = 1>
@@ -173,7 +177,7 @@ __device__ __forceinline__ void stockham(float2* s, int t, const float2* w) {
 template <int N, int E, bool INV>
 __device__ __forceinline__ void fft_lds(float2* s, int t, const FftTw<N, E>& tw) {
   static_assert((N & (N - 1)) == 0 && N >= 2, "N must be a power of two");
-  static_assert(E <= N && (E == 2 || E == 4 || E == 8), "E must be 2, 4 or 8");
+  static_assert(E <= N && (E == 2 || E == 4 || E == 8 || E == 16 || E == 32), "E must be 2..32");
   stockham<N, E, 1, 0, INV>(s, t, tw.w);
 }
 

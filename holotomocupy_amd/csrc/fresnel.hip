@@ -154,10 +154,17 @@ hipError_t launch_fresnel_rows(int64_t n, int64_t nz, const float2* in, int64_t 
 #undef HTAMD_ROWS_CASE
 }
 
+// Column-pass elements per thread: 32 at N = 2048 (64 threads per column, so a
+// 4-column tile is one 256-thread workgroup), else 8.
+template <int N>
+constexpr int col_e() {
+  return N >= 2048 ? N / 64 : 8;
+}
+
 template <int N, int C>
 static hipError_t cols_nc(int64_t n, float2* data, int64_t stride, int64_t batch, const float2* tw,
                           const float2* tab, hipStream_t st) {
-  constexpr int E = 8;
+  constexpr int E = col_e<N>();
   const int64_t tiles_per_field = n / C;
   const int64_t grid = batch * tiles_per_field;
   if (grid == 0) return hipSuccess;
@@ -166,20 +173,20 @@ static hipError_t cols_nc(int64_t n, float2* data, int64_t stride, int64_t batch
   return hipGetLastError();
 }
 
-// Columns per workgroup: as many as fit 1024 threads and ~64 KB of LDS, and
-// divide the row length n (n is even, so C = 2 always divides).
+// Columns per workgroup: up to 16, bounded by 256 threads and ~64 KB of LDS,
+// and dividing the row length n.
 template <int N>
 static hipError_t cols_n(int64_t n, float2* data, int64_t stride, int64_t batch, const float2* tw,
                          const float2* tab, hipStream_t st) {
-  constexpr int T = N / 8;
-  constexpr int CT = 1024 / T;
+  constexpr int T = N / col_e<N>();
+  constexpr int CT = 256 / T;
   constexpr int CL = 8192 / N;
   constexpr int CMAX0 = CT < CL ? CT : CL;
-  constexpr int CMAX = CMAX0 > 16 ? 16 : CMAX0;
+  constexpr int CMAX = CMAX0 > 16 ? 16 : (CMAX0 < 1 ? 1 : CMAX0);
   if (CMAX >= 16 && n % 16 == 0) return cols_nc<N, (CMAX >= 16 ? 16 : 1)>(n, data, stride, batch, tw, tab, st);
   if (CMAX >= 8 && n % 8 == 0) return cols_nc<N, (CMAX >= 8 ? 8 : 1)>(n, data, stride, batch, tw, tab, st);
   if (CMAX >= 4 && n % 4 == 0) return cols_nc<N, (CMAX >= 4 ? 4 : 1)>(n, data, stride, batch, tw, tab, st);
-  if (n % 2 == 0) return cols_nc<N, 2>(n, data, stride, batch, tw, tab, st);
+  if (CMAX >= 2 && n % 2 == 0) return cols_nc<N, (CMAX >= 2 ? 2 : 1)>(n, data, stride, batch, tw, tab, st);
   return cols_nc<N, 1>(n, data, stride, batch, tw, tab, st);
 }
 

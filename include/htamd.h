@@ -63,6 +63,13 @@ int htamd_prop_apply(const htamd_prop_plan* plan, const void* in, void* out, int
 int htamd_prop_apply_strided(const htamd_prop_plan* plan, const void* in, int64_t in_stride, void* out,
                              int64_t out_stride, int64_t batch, int64_t j, int adjoint, void* stream);
 
+/* One axis of D_j / DT_j: axis 0 = the row (x) pass in -> out, axis 1 = the
+ * column (y) pass, in place (in must equal out).  D_j = pass(axis 1) after
+ * pass(axis 0); the two passes commute.  Replaces the row / column stages of
+ * conv2d_run (src/holotomocupy/cuda/conv2d.cu:144-154). */
+int htamd_prop_pass(const htamd_prop_plan* plan, const void* in, int64_t in_stride, void* out,
+                    int64_t out_stride, int64_t batch, int64_t j, int adjoint, int axis, void* stream);
+
 /* Replaces conv2d_destroy (src/holotomocupy/cuda/conv2d.cu:157-159). */
 int htamd_prop_plan_destroy(htamd_prop_plan* plan);
 
