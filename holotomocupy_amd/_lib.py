@@ -31,6 +31,10 @@ SIGNATURES = {
     "htamd_cas_probe_grad": [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_int, c_vp],
     "htamd_cas_reduce_workspace_size": [c_i64, ctypes.POINTER(c_i64)],
     "htamd_cas_reduce": [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp],
+    "htamd_bh_plan_create": [c_vp, c_vp, ctypes.POINTER(c_vp)],
+    "htamd_bh_plan_destroy": [c_vp],
+    "htamd_bh_workspace_size": [c_vp, c_i64, ctypes.POINTER(c_i64)],
+    "htamd_bh_gradient": [c_vp, c_i64] + [c_vp] * 8 + [c_int] + [c_vp] * 5 + [c_i64, c_vp],
     "htamd_shift_plan_create": [c_i64, c_i64, c_i64, c_i64, c_int, ctypes.POINTER(c_vp)],
     "htamd_shift_plan_destroy": [c_vp],
     "htamd_bspline_prefilter": [c_vp, c_vp, c_vp, c_i64, c_vp],

@@ -1,0 +1,73 @@
+"""Fused BH gradient on gfx950 (the served path of the gradient evaluation).
+
+``BHGradient(rec)`` binds the propagator and shift plans of a ``Rec`` and
+evaluates, for a chunk of angles, exactly the quantities of
+Rec.gradients_cascade (src/holotomocupy/rec_mpi.py:327-353) through the C ABI
+``htamd_bh_gradient``: three FFT passes per (angle, distance), the F3 recompute
+done once per pixel, deterministic reductions.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+
+
+class BHGradient:
+    def __init__(self, rec):
+        self.rec = rec
+        self.device = rec.device
+        plan = ctypes.c_void_p()
+        _lib.call("htamd_bh_plan_create", rec.cl_prop._plan, rec.cl_shift._plan, ctypes.byref(plan))
+        self._plan = plan
+        self._ws = None
+        self._scales = {}
+
+    def __del__(self):
+        plan = getattr(self, "_plan", None)
+        if plan is not None and plan.value and _lib._lib is not None:
+            _lib._lib.htamd_bh_plan_destroy(plan)
+            self._plan = None
+
+    def workspace_bytes(self, nangles):
+        nb = ctypes.c_int64()
+        _lib.call("htamd_bh_workspace_size", self._plan, int(nangles), ctypes.byref(nb))
+        return nb.value
+
+    def workspace(self, nangles):
+        nb = self.workspace_bytes(nangles)
+        if self._ws is None or self._ws.numel() < nb:
+            self._ws = torch.empty(nb, dtype=torch.uint8, device=self.device)
+        return self._ws
+
+    def scales(self, nangles):
+        """Per-angle (2/data_size, rho_obj^2, rho_pos^2, 0) and per-probe rho_prb^2."""
+        key = nangles
+        if key not in self._scales:
+            r = self.rec
+            s = torch.tensor([2.0 / r.data_size, r.rho_sq["obj"], r.rho_sq["pos"], 0.0], dtype=torch.float32)
+            self._scales[key] = (s.repeat(nangles, 1).to(self.device),
+                                 torch.tensor([r.rho_sq["prb"]], dtype=torch.float32, device=self.device))
+        return self._scales[key]
+
+    def __call__(self, prb, proj, pos, data, mag, gproj, gpos, gprb, accumulate_prb=True,
+                 objective=None, probe_index=None, scales=None, rho_prb=None, workspace=None):
+        """Write gproj / gpos and accumulate (or overwrite) gprb for the angles
+        of proj; all arguments are device tensors with the layouts of
+        include/htamd.h.  objective: optional float64 device tensor [1]."""
+        B = proj.shape[0]
+        for t, name in ((prb, "prb"), (proj, "proj"), (pos, "pos"), (data, "data"), (mag, "mag"),
+                        (gproj, "gproj"), (gpos, "gpos"), (gprb, "gprb")):
+            if not t.is_contiguous() or t.device != self.device:
+                raise ValueError(f"{name} must be a contiguous tensor on {self.device}")
+        if scales is None or rho_prb is None:
+            s, rp = self.scales(B)
+            scales = s if scales is None else scales
+            rho_prb = rp if rho_prb is None else rho_prb
+        ws = self.workspace(B) if workspace is None else workspace
+        _lib.call("htamd_bh_gradient", self._plan, B, _lib.ptr(proj), _lib.ptr(pos), _lib.ptr(mag),
+                  _lib.ptr(prb), _lib.ptr(probe_index) if probe_index is not None else None,
+                  _lib.ptr(data), _lib.ptr(scales), _lib.ptr(rho_prb), 1 if accumulate_prb else 0,
+                  _lib.ptr(gproj), _lib.ptr(gpos), _lib.ptr(gprb),
+                  _lib.ptr(objective) if objective is not None else None,
+                  _lib.ptr(ws), ws.numel(), _lib.stream_ptr())

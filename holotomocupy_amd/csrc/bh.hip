@@ -1,0 +1,624 @@
+// Fused BH gradient of the holotomography data term on gfx950.
+//
+// Computes what Rec.gradients_cascade computes for a chunk of angles
+// (src/holotomocupy/rec_mpi.py:327-353, cascade 517-813):
+//   x22 = S_j(coeff(proj)), x12 = exp(i x22), x0 = D_j(prb_j x12),
+//   y0  = 2/N (x0 - d x0/|x0|), y12 = DT_j(y0),
+//   gprb_j  = sum_angles y12 conj(x12),   y22 = -i y12 conj(prb_j) conj(x12),
+//   (dpsi, dr) = dSadj(coeff(proj), y22),  gproj = coeff(sum_j dpsi),  gpos = dr
+// (each scaled by its rho^2), with the F3 recompute of the reference done once
+// per pixel and no [angles, ndist, nz, n] intermediate other than one work
+// field.  Passes per (angle, distance), all HBM-streaming:
+//   K1 rows:    S-gather + exp + probe product, D_x                 -> T
+//   K2 columns: D_y, residual against sqrt(data), DT_y (objective)  -> T
+//   K3 rows:    DT_x, probe-gradient accumulation over angles,
+//               y22 and the shift-gradient partial sums            -> T
+//   K4 object:  gather-form Sadj summed over distances              -> dpsi
+// then the B-spline prefilter of dpsi and two small deterministic reductions.
+#include "../../include/htamd.h"
+#include "fresnel_core.hpp"
+#include "htamd_bspline.hpp"
+#include "htamd_internal.hpp"
+
+namespace htamd {
+
+struct BhArgs {
+  int n = 0, nz = 0, npsi = 0, nzpsi = 0, ndist = 0;
+  int64_t nang = 0;
+  const void* c = nullptr;           // prefiltered projections [nang, nzpsi, npsi] (object type)
+  const float* pos = nullptr;        // [nang, ndist, 2] (y, x)
+  const float* mag = nullptr;        // [nang, ndist]
+  const float2* prb = nullptr;       // [nprb, ndist, nz, n]
+  const int* pidx = nullptr;         // [nang] probe of each angle (NULL: probe 0)
+  const float* data = nullptr;       // [nang, ndist, nz, n] sqrt(intensity)
+  const float4* scales = nullptr;    // [nang]: (2/data_size, rho_obj^2, rho_pos^2, unused)
+  float2* T = nullptr;               // work field [nang, ndist, nz, n]
+  float2* gprb = nullptr;            // [nprb, ndist, nz, n]
+  const float* rho_prb = nullptr;    // [nprb] rho_prb^2
+  int accumulate_prb = 0;
+  float2* part_r = nullptr;          // [nang*ndist*nz] shift-gradient partials
+  double* part_obj = nullptr;        // [column tiles] objective partials
+  const float2* twx = nullptr;
+  const float2* twy = nullptr;
+  const float2* tabx = nullptr;      // [ndist][2][3*(n/2)]
+  const float2* taby = nullptr;      // [ndist][2][3*(nz/2)]
+};
+
+template <typename TO> __device__ __forceinline__ float2 to_c(TO v);
+template <> __device__ __forceinline__ float2 to_c<float>(float v) { return make_float2(v, 0.f); }
+template <> __device__ __forceinline__ float2 to_c<float2>(float2 v) { return v; }
+template <typename TO> __device__ __forceinline__ TO from_c(float2 v);
+template <> __device__ __forceinline__ float from_c<float>(float2 v) { return v.x; }
+template <> __device__ __forceinline__ float2 from_c<float2>(float2 v) { return v; }
+
+// 16-tap B-spline gather at sampling point x of a row whose y-taps start at iy
+// with weights wy (and dwy).  v = sum wy wx c; with DERIV also
+// dt1 = -sum dwy wx c and dt2 = -sum wy dwx c (the dsadj weights,
+// cuda_kernels.py:735-748).
+template <typename TO, bool DERIV>
+__device__ __forceinline__ void s_gather(const TO* __restrict__ c, int npsi, int nzpsi, float x, int iy,
+                                         const float* wy, const float* dwy, TO& v, TO& dt1, TO& dt2) {
+  const float fx = floorf(x);
+  const int ix = (int)fx - 1;
+  float wx[4], dwx[4];
+  bs_w(x - fx, wx);
+  if (DERIV) bs_dw(x - fx, dwx);
+  v = vzero<TO>();
+  dt1 = vzero<TO>();
+  dt2 = vzero<TO>();
+#pragma unroll
+  for (int jy = 0; jy < 4; ++jy) {
+    const int yy = iy + jy;
+    if (yy < 0 || yy >= nzpsi) continue;
+    TO rv = vzero<TO>(), rd = vzero<TO>();
+#pragma unroll
+    for (int jx = 0; jx < 4; ++jx) {
+      const int xx = ix + jx;
+      if (xx < 0 || xx >= npsi) continue;
+      const TO cv = c[(int64_t)yy * npsi + xx];
+      rv = vfma(wx[jx], cv, rv);
+      if (DERIV) rd = vfma(dwx[jx], cv, rd);
+    }
+    v = vfma(wy[jy], rv, v);
+    if (DERIV) {
+      dt1 = vfma(-dwy[jy], rv, dt1);
+      dt2 = vfma(-wy[jy], rd, dt2);
+    }
+  }
+}
+
+// Sum of (s1, s2) over the T threads of one row group; every thread calls it.
+// The sum is returned to the row's thread t == 0 (and, for T <= 64, to all).
+template <int T>
+__device__ __forceinline__ float2 row_sum2(float2 v, float2* red) {
+  constexpr int W = T < 64 ? T : 64;
+#pragma unroll
+  for (int off = W / 2; off > 0; off >>= 1) {
+    v.x += __shfl_xor(v.x, off, 64);
+    v.y += __shfl_xor(v.y, off, 64);
+  }
+  if constexpr (T > 64) {
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x % T == 0) {
+      const int w0 = threadIdx.x >> 6;
+      v = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < T / 64; ++k) {
+        v.x += red[w0 + k].x;
+        v.y += red[w0 + k].y;
+      }
+    }
+  }
+  return v;
+}
+
+// ---- K1: rows, forward ------------------------------------------------------
+// Workgroup = ROWS row units (j, ty); loops over the angles of the launch.
+template <int N, int E, typename TO>
+__global__ __launch_bounds__(kRowThreads) void bh_fwd_rows_k(BhArgs a) {
+  constexpr int T = N / E, ROWS = kRowThreads / T;
+  __shared__ float2 lds[ROWS * N];
+  const int r = threadIdx.x / T, t = threadIdx.x % T;
+  const int64_t nunits = (int64_t)a.ndist * a.nz;
+  const int64_t unit = (int64_t)blockIdx.x * ROWS + r;
+  const bool valid = unit < nunits;
+  const int j = valid ? (int)(unit / a.nz) : 0, ty = valid ? (int)(unit % a.nz) : 0;
+  float2* s = lds + r * N;
+  FftTw<N, E> twr;
+  fft_load_twiddles<N, E>(twr, t, a.twx);
+  const float2* tab = a.tabx + (int64_t)(j * 2 + 0) * 3 * (N / 2);
+  const int64_t plane = (int64_t)a.nzpsi * a.npsi;
+  for (int64_t b = 0; b < a.nang; ++b) {
+    const int64_t bj = b * a.ndist + j;
+    const float mb = a.mag[bj], ry = a.pos[2 * bj], rx = a.pos[2 * bj + 1];
+    const float y = bs_coord(ty, a.nz, a.nzpsi, mb, ry);
+    const float fy = floorf(y);
+    const int iy = (int)fy - 1;
+    float wy[4];
+    bs_w(y - fy, wy);
+    const TO* cb = (const TO*)a.c + b * plane;
+    const int p = a.pidx ? a.pidx[b] : 0;
+    const float2* prow = a.prb + (((int64_t)p * a.ndist + j) * a.nz + ty) * N;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = t + e * T;
+      TO v, d1, d2;
+      s_gather<TO, false>(cb, a.npsi, a.nzpsi, bs_coord(i, N, a.npsi, mb, rx), iy, wy, wy, v, d1, d2);
+      s[dct_slot<N>(i)] = cmul(prow[i], expi(to_c(v)));
+    }
+    __syncthreads();
+    fresnel_1d<N, E>(s, t, twr, tab);
+    if (valid) {
+      float2* dst = a.T + (bj * a.nz + ty) * (int64_t)N;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int i = t + e * T;
+        dst[i] = s[dct_slot<N>(i)];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---- K2: columns: D_y, residual, DT_y ---------------------------------------
+template <int N, int E, int C>
+__global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_field) {
+  constexpr int T = N / E, NT = C * T, LS = N + 1;
+  __shared__ float2 lds[C * LS];
+  __shared__ double red[NT / 64 > 0 ? NT / 64 : 1];
+  const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t f = tile / tiles_per_field;  // field b*ndist + j
+  const int x0 = (int)(tile % tiles_per_field) * C;
+  const int64_t b = f / a.ndist;
+  const int j = (int)(f % a.ndist);
+  const int cc = threadIdx.x / T, t = threadIdx.x % T;
+  FftTw<N, E> twr;
+  fft_load_twiddles<N, E>(twr, t, a.twy);
+  float2* base = a.T + f * (int64_t)N * a.n + x0;
+  const float* dbase = a.data + f * (int64_t)N * a.n + x0;
+  float dv[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) dv[e] = dbase[(int64_t)dct_nat<N>(t + e * T) * a.n + cc];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int idx = threadIdx.x + e * NT;
+    const int c = idx % C, y = idx / C;
+    lds[c * LS + dct_slot<N>(y)] = base[(int64_t)y * a.n + c];
+  }
+  __syncthreads();
+  float2* s = lds + cc * LS;
+  fresnel_1d<N, E>(s, t, twr, a.taby + (int64_t)(j * 2 + 0) * 3 * (N / 2));
+  const float sc = a.scales[b].x;
+  double obj = 0.0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int p = t + e * T;
+    const float2 v = s[p];
+    const float am = sqrtf(v.x * v.x + v.y * v.y);
+    const float q = am - dv[e];
+    obj += (double)(q * q);
+    const float g = sc * (1.0f - dv[e] / am);
+    s[p] = make_float2(g * v.x, g * v.y);
+  }
+  __syncthreads();
+  fresnel_1d<N, E>(s, t, twr, a.taby + (int64_t)(j * 2 + 1) * 3 * (N / 2));
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int idx = threadIdx.x + e * NT;
+    const int c = idx % C, y = idx / C;
+    base[(int64_t)y * a.n + c] = lds[c * LS + dct_slot<N>(y)];
+  }
+  if constexpr (NT >= 64) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) obj += __shfl_xor(obj, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = obj;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double tot = 0.0;
+#pragma unroll
+      for (int k = 0; k < NT / 64; ++k) tot += red[k];
+      a.part_obj[tile] = tot;
+    }
+  } else {
+    __shared__ double all[NT];
+    all[threadIdx.x] = obj;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double tot = 0.0;
+      for (int k = 0; k < NT; ++k) tot += all[k];
+      a.part_obj[tile] = tot;
+    }
+  }
+}
+
+// ---- K3: rows, adjoint + epilogue ------------------------------------------
+template <int N, int E, typename TO>
+__global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
+  constexpr int T = N / E, ROWS = kRowThreads / T;
+  __shared__ float2 lds[ROWS * N];
+  __shared__ float2 red[kRowThreads / 64];
+  const int r = threadIdx.x / T, t = threadIdx.x % T;
+  const int64_t nunits = (int64_t)a.ndist * a.nz;
+  const int64_t unit = (int64_t)blockIdx.x * ROWS + r;
+  const bool valid = unit < nunits;
+  const int j = valid ? (int)(unit / a.nz) : 0, ty = valid ? (int)(unit % a.nz) : 0;
+  float2* s = lds + r * N;
+  FftTw<N, E> twr;
+  fft_load_twiddles<N, E>(twr, t, a.twx);
+  const float2* tab = a.tabx + (int64_t)(j * 2 + 1) * 3 * (N / 2);
+  const int64_t plane = (int64_t)a.nzpsi * a.npsi;
+  float2 acc[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) acc[e] = make_float2(0.f, 0.f);
+  int pcur = a.pidx ? a.pidx[0] : 0;
+  auto flush = [&](int p) {
+    if (!valid) return;
+    float2* g = a.gprb + (((int64_t)p * a.ndist + j) * a.nz + ty) * N;
+    const float rho = a.rho_prb[p];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = t + e * T;
+      float2 v = cscale(acc[e], rho);
+      if (a.accumulate_prb) v = cadd(g[i], v);
+      g[i] = v;
+      acc[e] = make_float2(0.f, 0.f);
+    }
+  };
+  for (int64_t b = 0; b < a.nang; ++b) {
+    const int64_t bj = b * a.ndist + j;
+    float2* row = a.T + (bj * a.nz + ty) * (int64_t)N;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = t + e * T;
+      s[dct_slot<N>(i)] = valid ? row[i] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fresnel_1d<N, E>(s, t, twr, tab);
+    const int p = a.pidx ? a.pidx[b] : 0;
+    if (p != pcur) {
+      flush(pcur);
+      pcur = p;
+    }
+    const float mb = a.mag[bj], ry = a.pos[2 * bj], rx = a.pos[2 * bj + 1];
+    const float y = bs_coord(ty, a.nz, a.nzpsi, mb, ry);
+    const float fy = floorf(y);
+    const int iy = (int)fy - 1;
+    float wy[4], dwy[4];
+    bs_w(y - fy, wy);
+    bs_dw(y - fy, dwy);
+    const TO* cb = (const TO*)a.c + b * plane;
+    const float2* prow = a.prb + (((int64_t)p * a.ndist + j) * a.nz + ty) * N;
+    float s1 = 0.f, s2 = 0.f;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = t + e * T;
+      const float2 y12 = s[dct_slot<N>(i)];
+      TO v, d1, d2;
+      s_gather<TO, true>(cb, a.npsi, a.nzpsi, bs_coord(i, N, a.npsi, mb, rx), iy, wy, dwy, v, d1, d2);
+      const float2 ex = expi(to_c(v));
+      acc[e] = cadd(acc[e], cmulc(y12, ex));                // y12 conj(x12)
+      const float2 q = cmulc(cmulc(y12, prow[i]), ex);      // y12 conj(prb) conj(x12)
+      const TO g = from_c<TO>(make_float2(q.y, -q.x));      // * (-i)
+      s1 += vredot(g, d1);
+      s2 += vredot(g, d2);
+      if (valid) row[i] = to_c(g);
+    }
+    const float2 tot = row_sum2<T>(make_float2(s1, s2), red);
+    if (valid && t == 0) a.part_r[bj * a.nz + ty] = tot;
+    __syncthreads();
+  }
+  flush(pcur);
+}
+
+// ---- K4: gather-form Sadj summed over distances ----------------------------
+template <typename TO>
+__global__ __launch_bounds__(256) void bh_sadj_k(BhArgs a, TO* __restrict__ dpsi, int tiles_x,
+                                                 int tiles_y) {
+  constexpr int TX = 64, TY = 4;
+  const int64_t bid = blockIdx.x;
+  const int txt = (int)(bid % tiles_x);
+  const int64_t rest = bid / tiles_x;
+  const int tyt = (int)(rest % tiles_y);
+  const int64_t b = rest / tiles_y;
+  const int ix = txt * TX + (int)(threadIdx.x % TX);
+  const int iy = tyt * TY + (int)(threadIdx.x / TX);
+  if (ix >= a.npsi || iy >= a.nzpsi) return;
+  TO acc = vzero<TO>();
+  for (int j = 0; j < a.ndist; ++j) {
+    const int64_t bj = b * a.ndist + j;
+    const float mb = a.mag[bj], ry = a.pos[2 * bj], rx = a.pos[2 * bj + 1];
+    int xlo, xhi, ylo, yhi;
+    bs_candidates(ix, a.n, a.npsi, mb, rx, xlo, xhi);
+    bs_candidates(iy, a.nz, a.nzpsi, mb, ry, ylo, yhi);
+    const float2* g = a.T + bj * (int64_t)a.nz * a.n;
+    for (int ty = ylo; ty <= yhi; ++ty) {
+      const float wyv = bs_reach(bs_coord(ty, a.nz, a.nzpsi, mb, ry), iy);
+      if (wyv == 0.0f) continue;
+      TO rowv = vzero<TO>();
+      for (int tx = xlo; tx <= xhi; ++tx) {
+        const float wxv = bs_reach(bs_coord(tx, a.n, a.npsi, mb, rx), ix);
+        if (wxv != 0.0f) rowv = vfma(wxv, from_c<TO>(g[(int64_t)ty * a.n + tx]), rowv);
+      }
+      acc = vfma(wyv, rowv, acc);
+    }
+  }
+  const float rho = a.scales[b].y;
+  dpsi[(b * a.nzpsi + iy) * (int64_t)a.npsi + ix] = vfma(rho, acc, vzero<TO>());
+}
+
+// ---- small reductions -------------------------------------------------------
+__global__ __launch_bounds__(256) void bh_pos_final_k(const float2* __restrict__ part_r, int nz,
+                                                      int ndist, const float4* __restrict__ scales,
+                                                      float* __restrict__ gpos) {
+  const int64_t bj = blockIdx.x;
+  double s1 = 0.0, s2 = 0.0;
+  for (int ty = threadIdx.x; ty < nz; ty += 256) {
+    const float2 v = part_r[bj * nz + ty];
+    s1 += (double)v.x;
+    s2 += (double)v.y;
+  }
+  __shared__ double r1[256], r2[256];
+  r1[threadIdx.x] = s1;
+  r2[threadIdx.x] = s2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      r1[threadIdx.x] += r1[threadIdx.x + w];
+      r2[threadIdx.x] += r2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float rho = scales[bj / ndist].z;
+    gpos[2 * bj + 0] = (float)r1[0] * rho;
+    gpos[2 * bj + 1] = (float)r2[0] * rho;
+  }
+}
+
+__global__ __launch_bounds__(256) void bh_sum_k(const double* __restrict__ part, int64_t count,
+                                                double scale, double* __restrict__ out) {
+  __shared__ double r[256];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < count; i += 256) s += part[i];
+  r[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) r[threadIdx.x] += r[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = r[0] * scale;
+}
+
+// ---- host side --------------------------------------------------------------
+static int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
+
+struct BhLayout {
+  int64_t c = 0, T = 0, dpsi = 0, part_r = 0, part_obj = 0, total = 0;
+};
+
+template <int NZ>
+static int col_tiles_c(int64_t n) {
+  constexpr int TT = NZ / col_e<NZ>();
+  constexpr int CT = 256 / TT, CL = 8192 / NZ;
+  constexpr int CM0 = CT < CL ? CT : CL;
+  constexpr int CM = CM0 > 16 ? 16 : (CM0 < 1 ? 1 : CM0);
+  for (int c = CM; c > 1; c >>= 1)
+    if (n % c == 0) return c;
+  return 1;
+}
+
+static int cols_per_tile(int64_t nz, int64_t n) {
+  switch (nz) {
+    case 8: return col_tiles_c<8>(n);
+    case 16: return col_tiles_c<16>(n);
+    case 32: return col_tiles_c<32>(n);
+    case 64: return col_tiles_c<64>(n);
+    case 128: return col_tiles_c<128>(n);
+    case 256: return col_tiles_c<256>(n);
+    case 512: return col_tiles_c<512>(n);
+    case 1024: return col_tiles_c<1024>(n);
+    case 2048: return col_tiles_c<2048>(n);
+    default: return 1;
+  }
+}
+
+static BhLayout bh_layout(const htamd_prop_plan* pp, const htamd_shift_plan* sp, int64_t nang) {
+  BhLayout L;
+  const int64_t osz = sp->is_complex ? 8 : 4;
+  const int64_t plane = sp->nzpsi * sp->npsi;
+  const int64_t field = pp->n * pp->nz;
+  const int64_t ntiles = nang * pp->ndist * (pp->n / cols_per_tile(pp->nz, pp->n));
+  int64_t off = 0;
+  L.c = off;
+  off += align256(nang * plane * osz);
+  L.T = off;
+  off += align256(nang * pp->ndist * field * 8);
+  L.dpsi = off;
+  off += align256(nang * plane * osz);
+  L.part_r = off;
+  off += align256(nang * pp->ndist * pp->nz * 8);
+  L.part_obj = off;
+  off += align256(ntiles * 8);
+  L.total = off;
+  return L;
+}
+
+template <int N, typename TO>
+static hipError_t launch_rows(bool adjoint, const BhArgs& a, hipStream_t st) {
+  constexpr int E = 8;
+  constexpr int ROWS = kRowThreads / (N / E);
+  const int64_t units = (int64_t)a.ndist * a.nz;
+  const int64_t grid = (units + ROWS - 1) / ROWS;
+  if (adjoint)
+    hipLaunchKernelGGL((bh_adj_rows_k<N, E, TO>), dim3((unsigned)grid), dim3(kRowThreads), 0, st, a);
+  else
+    hipLaunchKernelGGL((bh_fwd_rows_k<N, E, TO>), dim3((unsigned)grid), dim3(kRowThreads), 0, st, a);
+  return hipGetLastError();
+}
+
+template <typename TO>
+static hipError_t rows_dispatch(bool adjoint, const BhArgs& a, hipStream_t st) {
+  switch (a.n) {
+    case 8: return launch_rows<8, TO>(adjoint, a, st);
+    case 16: return launch_rows<16, TO>(adjoint, a, st);
+    case 32: return launch_rows<32, TO>(adjoint, a, st);
+    case 64: return launch_rows<64, TO>(adjoint, a, st);
+    case 128: return launch_rows<128, TO>(adjoint, a, st);
+    case 256: return launch_rows<256, TO>(adjoint, a, st);
+    case 512: return launch_rows<512, TO>(adjoint, a, st);
+    case 1024: return launch_rows<1024, TO>(adjoint, a, st);
+    case 2048: return launch_rows<2048, TO>(adjoint, a, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int NZ>
+static hipError_t launch_cols(const BhArgs& a, hipStream_t st) {
+  constexpr int E = col_e<NZ>();
+  const int C = cols_per_tile(NZ, a.n);
+  const int tpf = a.n / C;
+  const int64_t grid = a.nang * a.ndist * tpf;
+  switch (C) {
+#define HTAMD_BHC(CC)                                                                                 \
+  case CC:                                                                                            \
+    if constexpr (CC * (NZ / E) <= 256 && CC * NZ <= 8192)                                           \
+      hipLaunchKernelGGL((bh_cols_k<NZ, E, CC>), dim3((unsigned)grid), dim3(CC * (NZ / E)), 0, st, a, \
+                         tpf);                                                                        \
+    break;
+    HTAMD_BHC(1) HTAMD_BHC(2) HTAMD_BHC(4) HTAMD_BHC(8) HTAMD_BHC(16)
+#undef HTAMD_BHC
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+static hipError_t cols_dispatch(const BhArgs& a, hipStream_t st) {
+  switch (a.nz) {
+    case 8: return launch_cols<8>(a, st);
+    case 16: return launch_cols<16>(a, st);
+    case 32: return launch_cols<32>(a, st);
+    case 64: return launch_cols<64>(a, st);
+    case 128: return launch_cols<128>(a, st);
+    case 256: return launch_cols<256>(a, st);
+    case 512: return launch_cols<512>(a, st);
+    case 1024: return launch_cols<1024>(a, st);
+    case 2048: return launch_cols<2048>(a, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename TO>
+static hipError_t bh_run(const htamd_shift_plan* sp, BhArgs a, const void* proj, void* gproj,
+                         float* gpos, double* objective, const BhLayout& L, char* ws, hipStream_t st) {
+  hipError_t e;
+  // coefficients of the projections (coeff, shift.py:74-82)
+  if ((e = bs_launch_prefilter<TO>((const TO*)proj, (TO*)(ws + L.c), a.nang, a.nzpsi, a.npsi, sp->taps,
+                                   st)) != hipSuccess)
+    return e;
+  if ((e = rows_dispatch<TO>(false, a, st)) != hipSuccess) return e;
+  if ((e = cols_dispatch(a, st)) != hipSuccess) return e;
+  if ((e = rows_dispatch<TO>(true, a, st)) != hipSuccess) return e;
+  constexpr int TX = 64, TY = 4;
+  const int tx = (a.npsi + TX - 1) / TX, ty = (a.nzpsi + TY - 1) / TY;
+  TO* dpsi = (TO*)(ws + L.dpsi);
+  hipLaunchKernelGGL((bh_sadj_k<TO>), dim3((unsigned)(a.nang * tx * ty)), dim3(256), 0, st, a, dpsi, tx, ty);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = bs_launch_prefilter<TO>(dpsi, (TO*)gproj, a.nang, a.nzpsi, a.npsi, sp->taps, st)) != hipSuccess)
+    return e;
+  hipLaunchKernelGGL(bh_pos_final_k, dim3((unsigned)(a.nang * a.ndist)), dim3(256), 0, st, a.part_r, a.nz,
+                     a.ndist, a.scales, gpos);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (objective) {
+    const int64_t ntiles = a.nang * a.ndist * (a.n / cols_per_tile(a.nz, a.n));
+    hipLaunchKernelGGL(bh_sum_k, dim3(1), dim3(256), 0, st, a.part_obj, ntiles, 1.0, objective);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+}  // namespace htamd
+
+using namespace htamd;
+
+struct htamd_bh_plan {
+  const htamd_prop_plan* prop = nullptr;
+  const htamd_shift_plan* shift = nullptr;
+};
+
+extern "C" {
+
+int htamd_bh_plan_create(const htamd_prop_plan* prop, const htamd_shift_plan* shift,
+                         htamd_bh_plan** plan) {
+  if (!plan || !prop || !shift) return fail(HTAMD_ERR_ARG, "bh_plan_create: NULL argument");
+  *plan = nullptr;
+  if (prop->n != shift->n || prop->nz != shift->nz)
+    return fail(HTAMD_ERR_ARG, "bh_plan_create: propagator and shift plans disagree on the detector size");
+  if ((shift->npsi % 2) || (shift->nzpsi % 2))
+    return fail(HTAMD_ERR_UNSUPPORTED, "bh_plan_create: object grid sizes must be even");
+  auto* p = new htamd_bh_plan();
+  p->prop = prop;
+  p->shift = shift;
+  *plan = p;
+  return HTAMD_OK;
+}
+
+int htamd_bh_plan_destroy(htamd_bh_plan* p) {
+  delete p;
+  return HTAMD_OK;
+}
+
+int htamd_bh_workspace_size(const htamd_bh_plan* p, int64_t nangles, int64_t* bytes) {
+  if (!p || !bytes || nangles < 0) return fail(HTAMD_ERR_ARG, "bh_workspace_size: bad arguments");
+  *bytes = bh_layout(p->prop, p->shift, nangles).total;
+  return HTAMD_OK;
+}
+
+int htamd_bh_gradient(const htamd_bh_plan* p, int64_t nangles, const void* proj, const float* pos,
+                      const float* mag, const void* prb, const int* probe_index, const float* data,
+                      const float* angle_scales, const float* rho_prb, int accumulate_prb, void* gproj,
+                      float* gpos, void* gprb, double* objective, void* workspace,
+                      int64_t workspace_bytes, void* stream) {
+  if (!p) return fail(HTAMD_ERR_ARG, "bh_gradient: plan is NULL");
+  if (nangles < 0) return fail(HTAMD_ERR_ARG, "bh_gradient: negative angle count");
+  if (nangles == 0) return HTAMD_OK;
+  if (!proj || !pos || !mag || !prb || !data || !angle_scales || !rho_prb || !gproj || !gpos || !gprb)
+    return fail(HTAMD_ERR_ARG, "bh_gradient: NULL buffer");
+  const BhLayout L = bh_layout(p->prop, p->shift, nangles);
+  if (!workspace || workspace_bytes < L.total)
+    return fail(HTAMD_ERR_ARG, "bh_gradient: workspace of " + std::to_string(L.total) + " bytes required");
+  int dev = -1;
+  (void)hipGetDevice(&dev);
+  if (dev != p->prop->device) return fail(HTAMD_ERR_ARG, "bh_gradient: plan belongs to another device");
+  char* ws = (char*)workspace;
+  BhArgs a;
+  a.n = (int)p->prop->n;
+  a.nz = (int)p->prop->nz;
+  a.npsi = (int)p->shift->npsi;
+  a.nzpsi = (int)p->shift->nzpsi;
+  a.ndist = (int)p->prop->ndist;
+  a.nang = nangles;
+  a.c = ws + L.c;
+  a.pos = pos;
+  a.mag = mag;
+  a.prb = (const float2*)prb;
+  a.pidx = probe_index;
+  a.data = data;
+  a.scales = (const float4*)angle_scales;
+  a.T = (float2*)(ws + L.T);
+  a.gprb = (float2*)gprb;
+  a.rho_prb = rho_prb;
+  a.accumulate_prb = accumulate_prb;
+  a.part_r = (float2*)(ws + L.part_r);
+  a.part_obj = (double*)(ws + L.part_obj);
+  a.twx = p->prop->tw_x;
+  a.twy = p->prop->tw_y;
+  a.tabx = p->prop->tab_x;
+  a.taby = p->prop->tab_y;
+  hipError_t e = p->shift->is_complex
+      ? bh_run<float2>(p->shift, a, proj, gproj, gpos, objective, L, ws, (hipStream_t)stream)
+      : bh_run<float>(p->shift, a, proj, gproj, gpos, objective, L, ws, (hipStream_t)stream);
+  return e == hipSuccess ? HTAMD_OK : hip_fail(e, "bh_gradient");
+}
+
+}  // extern "C"

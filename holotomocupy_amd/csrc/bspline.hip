@@ -114,27 +114,6 @@ __global__ __launch_bounds__(kTX* kTY) void bs_forward_k(BsArgs<T> a) {
 // Candidate pixels are bracketed analytically with a one-pixel margin and each
 // is re-evaluated with the forward's exact float expression, so the weights
 // are bitwise those of S.
-__device__ __forceinline__ void bs_candidates(int i, int npix, int npsi, float mb, float rr, int& lo,
-                                              int& hi) {
-  const float off = bs_coord(0, npix, npsi, mb, rr) + mb * (float)(npix / 2);
-  const float a = ((float)i - 2.0f - off) / mb + (float)(npix / 2);
-  const float bnd = ((float)i + 2.0f - off) / mb + (float)(npix / 2);
-  lo = (int)floorf(fminf(a, bnd)) - 1;
-  hi = (int)ceilf(fmaxf(a, bnd)) + 1;
-  lo = lo < 0 ? 0 : lo;
-  hi = hi > npix - 1 ? npix - 1 : hi;
-}
-
-// weight with which pixel coordinate `x` reaches coefficient index i (0 if none)
-__device__ __forceinline__ float bs_reach(float x, int i) {
-  const float fx = floorf(x);
-  const int j = i - (int)fx;  // tap offset, valid in [-1, 2]
-  if (j < -1 || j > 2) return 0.0f;
-  float w[4];
-  bs_w(x - fx, w);
-  return w[j + 1];
-}
-
 template <typename T>
 __global__ __launch_bounds__(kTX* kTY) void bs_adjoint_k(BsArgs<T> a) {
   // here a.c is the input g [B, nz, n] and a.out the coefficients [B, nzpsi, npsi]

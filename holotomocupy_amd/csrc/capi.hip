@@ -73,14 +73,7 @@ static hipError_t upload(T** dst, const std::vector<T>& src) {
 
 using namespace htamd;
 
-struct htamd_prop_plan {
-  int64_t n = 0, nz = 0, ndist = 0;
-  int device = -1;
-  float2* tw_x = nullptr;
-  float2* tw_y = nullptr;
-  float2* tab_x = nullptr;  // [ndist][2][3*(n/2)]
-  float2* tab_y = nullptr;  // [ndist][2][3*(nz/2)]
-};
+
 
 extern "C" {
 

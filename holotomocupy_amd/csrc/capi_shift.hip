@@ -11,12 +11,6 @@
 
 using namespace htamd;
 
-struct htamd_shift_plan {
-  int64_t n = 0, npsi = 0, nz = 0, nzpsi = 0;
-  int is_complex = 1;
-  PrefilterTaps taps{};
-};
-
 namespace htamd {
 
 // Periodic inverse of the [1 4 1] circulant of length N, i.e. the impulse

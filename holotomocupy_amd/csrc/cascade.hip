@@ -43,13 +43,6 @@ template <typename T> __device__ __forceinline__ float2 as_c(T v);
 template <> __device__ __forceinline__ float2 as_c<float>(float v) { return make_float2(v, 0.f); }
 template <> __device__ __forceinline__ float2 as_c<float2>(float2 v) { return v; }
 
-// exp(i x) for real or complex x
-__device__ __forceinline__ float2 expi(float2 x) {
-  float s, c;
-  sincosf(x.x, &s, &c);
-  const float a = expf(-x.y);
-  return make_float2(a * c, a * s);
-}
 
 // mode 0: out1 = e^{ix}; 1: out1 = e^{ix}, out2 = e^{ix} i y; 2: out1 = e^{ix}(-y z) [+ e^{ix} i w];
 // 3: out1 = -i y12 conj(e^{ix}) (TO = float keeps the real part)

@@ -188,4 +188,10 @@ __device__ __forceinline__ int dct_slot(int i) {
   return (i & 1) ? (N - 1 - (i >> 1)) : (i >> 1);
 }
 
+// Natural index of FFT slot p under that reordering (inverse of dct_slot).
+template <int N>
+__device__ __forceinline__ int dct_nat(int p) {
+  return p < N / 2 ? 2 * p : 2 * (N - 1 - p) + 1;
+}
+
 }  // namespace htamd

@@ -50,6 +50,30 @@ __device__ __forceinline__ float bs_coord(int t, int npix, int npsi, float m, fl
   return (m * (float)(t - npix / 2) - r + half) + (float)(npsi / 2);
 }
 
+// Sadj as a gather: candidate detector pixels [lo, hi] whose taps can reach
+// coefficient index i, bracketed analytically with a one-pixel margin (each is
+// then re-evaluated with the forward's exact float expression).
+__device__ __forceinline__ void bs_candidates(int i, int npix, int npsi, float mb, float rr, int& lo,
+                                              int& hi) {
+  const float off = bs_coord(0, npix, npsi, mb, rr) + mb * (float)(npix / 2);
+  const float a = ((float)i - 2.0f - off) / mb + (float)(npix / 2);
+  const float bnd = ((float)i + 2.0f - off) / mb + (float)(npix / 2);
+  lo = (int)floorf(fminf(a, bnd)) - 1;
+  hi = (int)ceilf(fmaxf(a, bnd)) + 1;
+  lo = lo < 0 ? 0 : lo;
+  hi = hi > npix - 1 ? npix - 1 : hi;
+}
+
+// weight with which pixel coordinate `x` reaches coefficient index i (0 if none)
+__device__ __forceinline__ float bs_reach(float x, int i) {
+  const float fx = floorf(x);
+  const int j = i - (int)fx;  // tap offset, valid in [-1, 2]
+  if (j < -1 || j > 2) return 0.0f;
+  float w[4];
+  bs_w(x - fx, w);
+  return w[j + 1];
+}
+
 // Block-wide sum of a float2 (fixed order: wave shuffles, then LDS).
 template <int NT>
 __device__ __forceinline__ float2 block_sum2(float2 v) {

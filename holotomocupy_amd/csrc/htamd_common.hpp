@@ -24,6 +24,14 @@ __device__ __forceinline__ float2 mul_mi(float2 a) {
   return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
 }
 
+// exp(i x) for real or complex x
+__device__ __forceinline__ float2 expi(float2 x) {
+  float s, c;
+  sincosf(x.x, &s, &c);
+  const float a = expf(-x.y);
+  return make_float2(a * c, a * s);
+}
+
 // Workgroup -> logical tile mapping that gives each XCD a contiguous range of
 // tiles, so tiles that share cache lines are resident in the same L2.
 __device__ __forceinline__ int64_t xcd_tile(int64_t bid, int64_t nblocks) {

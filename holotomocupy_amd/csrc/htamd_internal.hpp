@@ -4,6 +4,8 @@
 #include <stdint.h>
 #include <string>
 
+#include "htamd_bspline.hpp"
+
 namespace htamd {
 
 void set_error(const std::string& msg);
@@ -28,3 +30,20 @@ hipError_t launch_fresnel_cols(int64_t nz, int64_t n, float2* data, int64_t stri
                                const float2* tw, const float2* tab, hipStream_t st);
 
 }  // namespace htamd
+
+// Plan objects behind the opaque handles of include/htamd.h (device tables of
+// a propagator plan live on plan->device).
+struct htamd_prop_plan {
+  int64_t n = 0, nz = 0, ndist = 0;
+  int device = -1;
+  float2* tw_x = nullptr;
+  float2* tw_y = nullptr;
+  float2* tab_x = nullptr;  // [ndist][2][3*(n/2)]
+  float2* tab_y = nullptr;  // [ndist][2][3*(nz/2)]
+};
+
+struct htamd_shift_plan {
+  int64_t n = 0, npsi = 0, nz = 0, nzpsi = 0;
+  int is_complex = 1;
+  htamd::PrefilterTaps taps{};
+};

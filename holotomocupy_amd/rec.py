@@ -284,6 +284,20 @@ class Rec:
         grads["proj"][:] = y[1] * self.rho_sq["obj"]
         grads["pos"][:] = y[2] * self.rho_sq["pos"]
 
+    def gradients_fused(self, vars, grads, data=None, st=0, objective=None):
+        """Same result as gradients_cascade through the fused three-pass HIP
+        pipeline (holotomocupy_amd.bh); grads['prb'] is accumulated into.
+        objective: optional float64 [1] device tensor receiving
+        sum (|x0| - d)^2 over these angles."""
+        from .bh import BHGradient
+        data = self.data if data is None else data
+        B = vars["proj"].shape[0]
+        if getattr(self, "_bh", None) is None:
+            self._bh = BHGradient(self)
+        mag = self.eff_demagnifications[st:st + B].contiguous()
+        self._bh(vars["prb"], vars["proj"], vars["pos"], data, mag, grads["proj"], grads["pos"],
+                 grads["prb"], accumulate_prb=True, objective=objective)
+
     def hessian_cascade(self, vars, grads, etas, data=None, st=0):
         """BH bilinear form of the data term (rec_mpi.py:250-303), as a float."""
         data = self.data if data is None else data

@@ -173,6 +173,34 @@ int htamd_cas_reduce(int kind, const void* x, const float* d, const void* y, con
                      const void* w, int64_t count, double* result, void* workspace,
                      int64_t workspace_bytes, void* stream);
 
+/* ---- fused BH gradient ------------------------------------------------------
+ * One BH gradient evaluation of the data term for `nangles` angles, i.e. what
+ * Rec.gradients_cascade computes for a chunk (src/holotomocupy/rec_mpi.py:327-353
+ * with the cascade 517-813), in three FFT passes per (angle, distance) and no
+ * per-operator intermediates.  The plan binds a propagator plan (n, nz, ndist
+ * distances) and a shift plan (object grid nzpsi x npsi, object dtype); both
+ * must outlive it.  Inputs (device):
+ *   proj [nangles, nzpsi, npsi] object dtype, pos [nangles, ndist, 2] (y, x),
+ *   mag [nangles, ndist] (eff_demagnifications), prb [nprobes, ndist, nz, n],
+ *   probe_index [nangles] int32 (NULL: every angle uses probe 0; angles of one
+ *   probe must be contiguous unless accumulate_prb), data [nangles, ndist, nz, n]
+ *   float32 = sqrt(intensity), angle_scales [nangles, 4] float32 =
+ *   (2/data_size, rho_obj^2, rho_pos^2, 0), rho_prb [nprobes] = rho_prb^2.
+ * Outputs: gproj like proj, gpos like pos, gprb like prb (overwritten, or
+ * added to when accumulate_prb != 0), objective (nullable, device double) =
+ * sum (|x0| - d)^2 over the chunk.  Deterministic (no atomics). */
+typedef struct htamd_bh_plan htamd_bh_plan;
+int htamd_bh_plan_create(const htamd_prop_plan* prop, const htamd_shift_plan* shift, htamd_bh_plan** plan);
+int htamd_bh_plan_destroy(htamd_bh_plan* plan);
+int htamd_bh_workspace_size(const htamd_bh_plan* plan, int64_t nangles, int64_t* bytes);
+/* Replaces Rec.gradients_cascade / _gradients_cascade and the gF0..gF3 chain
+ * (src/holotomocupy/rec_mpi.py:327-353, 555-813). */
+int htamd_bh_gradient(const htamd_bh_plan* plan, int64_t nangles, const void* proj, const float* pos,
+                      const float* mag, const void* prb, const int* probe_index, const float* data,
+                      const float* angle_scales, const float* rho_prb, int accumulate_prb, void* gproj,
+                      float* gpos, void* gprb, double* objective, void* workspace,
+                      int64_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

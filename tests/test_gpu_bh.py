@@ -1,0 +1,113 @@
+"""GPU parity of the fused BH gradient (holotomocupy_amd.bh) against the FP64
+oracle cascade and the per-operator cascade.  Tolerances as for the cascade:
+relative L2 <= 1e-5 (object, probe), 1e-4 (positions, objective)."""
+import numpy as np
+import pytest
+import torch
+
+from test_gpu_rec import make_problem, dev, lst, rel
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("real", [False, True])
+@pytest.mark.parametrize("geom", [(32, 40, 3, 4), (64, 48, 2, 3), (16, 24, 4, 5)])
+def test_fused_gradient_vs_oracle(gpu, real, geom):
+    n, nobj, ndist, B = geom
+    print(f"[case] fused gradient n={n} nobj={nobj} ndist={ndist} B={B} real={real}", flush=True)
+    rec, cas, x, v, w, d = make_problem(real, n=n, nobj=nobj, ndist=ndist, B=B, seed=20 + n)
+    xd, dd = dev(x), torch.from_numpy(d).cuda()
+    grads = {"prb": torch.zeros_like(xd["prb"]), "proj": torch.empty_like(xd["proj"]),
+             "pos": torch.empty_like(xd["pos"])}
+    obj = torch.zeros(1, dtype=torch.float64, device="cuda")
+    rec.gradients_fused(xd, grads, dd, objective=obj)
+    gprb, gproj, gpos = cas.gradients(*lst(x), d)
+    assert rel(grads["prb"], gprb) < 1e-5
+    assert rel(grads["proj"], gproj) < 1e-5
+    assert rel(grads["pos"], gpos) < 1e-4
+    assert obj.item() / rec.data_size == pytest.approx(cas.objective(*lst(x), d), rel=1e-4)
+    # the per-operator cascade gives the same gradient
+    g2 = {"prb": torch.zeros_like(xd["prb"]), "proj": torch.empty_like(xd["proj"]),
+          "pos": torch.empty_like(xd["pos"])}
+    rec.gradients_cascade(xd, g2, dd)
+    assert rel(grads["proj"], g2["proj"].cpu().numpy()) < 1e-5
+    assert rel(grads["prb"], g2["prb"].cpu().numpy()) < 1e-5
+
+
+def test_fused_gradient_deterministic_and_accumulates(gpu):
+    print("[case] fused gradient determinism", flush=True)
+    rec, cas, x, v, w, d = make_problem(False, seed=31)
+    xd, dd = dev(x), torch.from_numpy(d).cuda()
+    outs = []
+    for _ in range(2):
+        g = {"prb": torch.zeros_like(xd["prb"]), "proj": torch.empty_like(xd["proj"]),
+             "pos": torch.empty_like(xd["pos"])}
+        rec.gradients_fused(xd, g, dd)
+        outs.append(g)
+    for k in outs[0]:
+        assert torch.equal(outs[0][k], outs[1][k]), k
+    # grads['prb'] is accumulated (the reference's +=, rec_mpi.py:347)
+    g = {"prb": outs[0]["prb"].clone(), "proj": torch.empty_like(xd["proj"]), "pos": torch.empty_like(xd["pos"])}
+    rec.gradients_fused(xd, g, dd)
+    assert torch.allclose(g["prb"], 2 * outs[0]["prb"], rtol=1e-6, atol=1e-12)
+
+
+def test_fused_gradient_two_probes(gpu):
+    """Angles of two requests with different probes in one launch: each probe
+    gradient equals the one computed separately."""
+    print("[case] fused gradient two probes", flush=True)
+    from holotomocupy_amd.bh import BHGradient
+    rec, cas, x, v, w, d = make_problem(False, seed=41, B=4)
+    bh = BHGradient(rec)
+    xd, dd = dev(x), torch.from_numpy(d).cuda()
+    B = xd["proj"].shape[0]
+    prb2 = torch.stack([xd["prb"], xd["prb"] * (0.8 + 0.1j)])        # [2, ndist, nz, n]
+    pidx = torch.tensor([0, 0, 1, 1], dtype=torch.int32, device="cuda")
+    mag = rec.eff_demagnifications[:B].contiguous()
+    gproj = torch.empty_like(xd["proj"])
+    gpos = torch.empty_like(xd["pos"])
+    gprb = torch.zeros_like(prb2)
+    s, rp = bh.scales(B)
+    bh(prb2, xd["proj"], xd["pos"], dd, mag, gproj, gpos, gprb, accumulate_prb=False,
+       probe_index=pidx, scales=s, rho_prb=rp.repeat(2))
+    for p, sl in ((0, slice(0, 2)), (1, slice(2, 4))):
+        gp = torch.empty_like(xd["proj"][sl])
+        gq = torch.empty_like(xd["pos"][sl])
+        gr = torch.zeros_like(xd["prb"])
+        bh(prb2[p].contiguous(), xd["proj"][sl].contiguous(), xd["pos"][sl].contiguous(), dd[sl].contiguous(),
+           mag[sl].contiguous(), gp, gq, gr, accumulate_prb=False, scales=s[sl].contiguous(), rho_prb=rp)
+        assert torch.allclose(gprb[p], gr, rtol=1e-5, atol=1e-7 * gr.abs().max().item())
+        assert torch.allclose(gproj[sl], gp, rtol=1e-5, atol=1e-7 * gp.abs().max().item())
+        assert torch.allclose(gpos[sl], gq, rtol=1e-5, atol=1e-7 * gq.abs().max().item())
+
+
+def test_fused_gradient_nonsquare_detector(gpu):
+    """nz != n and magnified shifts."""
+    print("[case] fused gradient non-square", flush=True)
+    from types import SimpleNamespace
+    from holotomocupy_amd.rec import Rec, physics
+    from oracle import cascade as oc
+    from oracle import propagation as op
+    from oracle import shift as osh
+    rng = np.random.default_rng(5)
+    n, nz, nobj, nzobj, ndist, B = 64, 32, 72, 40, 2, 3
+    args = SimpleNamespace(energy=17.1, detector_pixelsize=1.4760147601476e-6 * 16, focustodetectordistance=1.217,
+                           z1=np.array([5.110, 6.879]) * 1e-3, ndist=ndist, ntheta=B, nz=nz, n=n, nzobj=nzobj,
+                           nobj=nobj, obj_dtype="complex64", rho=[1, 0.5, 0.2])
+    rec = Rec(args)
+    ph = physics(args.energy, args.z1, args.focustodetectordistance, args.detector_pixelsize)
+    cas = oc.Cascade(op.Propagation(n, nz, B, ndist, ph.wavelength, ph.voxelsize, ph.distance),
+                     osh.Shift(n, nobj, nz, nzobj), ndist, nz, n, nzobj, nobj, rec.data_size,
+                     np.tile(1 / ph.norm_magnifications, (B, 1)), rho=args.rho)
+    prb = (1 + 0.1 * (rng.standard_normal((ndist, nz, n)) + 1j * rng.standard_normal((ndist, nz, n)))).astype(np.complex64)
+    proj = (0.3 * (rng.standard_normal((B, nzobj, nobj)) + 1j * rng.standard_normal((B, nzobj, nobj)))).astype(np.complex64)
+    pos = rng.uniform(-3, 3, (B, ndist, 2)).astype(np.float32)
+    d = (np.abs(cas.forward(prb.astype(np.complex128), proj.astype(np.complex128), pos.astype(np.float64))) * 1.1).astype(np.float32)
+    x = {"prb": prb, "proj": proj, "pos": pos}
+    xd, dd = dev(x), torch.from_numpy(d).cuda()
+    g = {"prb": torch.zeros_like(xd["prb"]), "proj": torch.empty_like(xd["proj"]), "pos": torch.empty_like(xd["pos"])}
+    rec.gradients_fused(xd, g, dd)
+    gprb, gproj, gpos = cas.gradients(*lst(x), d)
+    assert rel(g["prb"], gprb) < 1e-5
+    assert rel(g["proj"], gproj) < 1e-5
+    assert rel(g["pos"], gpos) < 1e-4
