@@ -33,7 +33,7 @@ def parse():
     p.add_argument("--n", type=int, default=2048)
     p.add_argument("--nobj", type=int, default=1024)
     p.add_argument("--ndist", type=int, default=4)
-    p.add_argument("--path", choices=["auto", "plain", "served"], default="auto")
+    p.add_argument("--path", choices=["auto", "plain", "fused", "served"], default="auto")
     p.add_argument("--no-cpu-baseline", action="store_true")
     return p.parse_args()
 
@@ -73,6 +73,11 @@ def make_inputs(a, seed=1004):
 def plain_step(rec, vars_, grads, data):
     grads["prb"].zero_()
     rec.gradients_cascade(vars_, grads, data)
+
+
+def fused_step(rec, vars_, grads, data):
+    grads["prb"].zero_()
+    rec.gradients_fused(vars_, grads, data)
 
 
 def time_steps(fn, steps, warmup):
@@ -160,13 +165,21 @@ def main():
     rec, vars_, data, host = make_inputs(a)
     grads = {"prb": torch.zeros_like(vars_["prb"]), "proj": torch.empty_like(vars_["proj"]),
              "pos": torch.empty_like(vars_["pos"])}
-    path = "plain" if a.path in ("auto", "plain") else "served"
-    if path != "plain":
+    path = "fused" if a.path == "auto" else a.path
+    if path == "served":
         raise SystemExit("served path not built yet")
-    fn = lambda: plain_step(rec, vars_, grads, data)  # noqa: E731
-    wall, per = time_steps(fn, a.steps, a.warmup)
+    fn_plain = lambda: plain_step(rec, vars_, grads, data)  # noqa: E731
+    if path == "plain":
+        wall, per = time_steps(fn_plain, a.steps, a.warmup)
+        plain_value = value = a.angles * a.steps / wall
+    else:
+        n_plain = max(2, a.steps // 2)
+        wall_p, _ = time_steps(fn_plain, n_plain, 1)
+        plain_value = a.angles * n_plain / wall_p
+        fn = lambda: fused_step(rec, vars_, grads, data)  # noqa: E731
+        wall, per = time_steps(fn, a.steps, a.warmup)
+        value = a.angles * a.steps / wall
     ms = wall * 1e3 / a.steps
-    value = a.angles * a.steps / wall
     roof = roofline_fresnel_cols(rec, a)
     out = {
         "metric": "BH gradient evals/s at 2048^2 (served vs plain), % HBM roofline, p50/p99 latency",
@@ -177,7 +190,7 @@ def main():
                                f"object plane {a.nobj}x{a.nobj}, {a.angles} angles per request",
                    "path": path, "angles_per_step": a.angles, "n": a.n, "ndist": a.ndist, "nobj": a.nobj},
         "latency_ms": {"p50": round(per[len(per) // 2], 3), "p99": round(per[min(len(per) - 1, int(0.99 * len(per)))], 3)},
-        "plain_value": round(value, 3),
+        "plain_value": round(plain_value, 3), "speedup_vs_plain": round(value / plain_value, 3),
         "roofline": roof,
     }
     if not a.no_cpu_baseline:

@@ -75,7 +75,7 @@ __device__ __forceinline__ void s_gather(const TO* __restrict__ c, int npsi, int
     for (int jx = 0; jx < 4; ++jx) {
       const int xx = ix + jx;
       if (xx < 0 || xx >= npsi) continue;
-      const TO cv = c[(int64_t)yy * npsi + xx];
+      const TO cv = c[yy * npsi + xx];
       rv = vfma(wx[jx], cv, rv);
       if (DERIV) rd = vfma(dwx[jx], cv, rd);
     }
@@ -119,17 +119,21 @@ template <int N, int E, typename TO>
 __global__ __launch_bounds__(kRowThreads) void bh_fwd_rows_k(BhArgs a) {
   constexpr int T = N / E, ROWS = kRowThreads / T;
   __shared__ float2 lds[ROWS * N];
-  const int r = threadIdx.x / T, t = threadIdx.x % T;
+  const int r = threadIdx.x / T, t0 = threadIdx.x % T;
   const int64_t nunits = (int64_t)a.ndist * a.nz;
-  const int64_t unit = (int64_t)blockIdx.x * ROWS + r;
+  const int64_t unit = xcd_tile(blockIdx.x, gridDim.x) * ROWS + r;  // XCD-contiguous row bands
   const bool valid = unit < nunits;
   const int j = valid ? (int)(unit / a.nz) : 0, ty = valid ? (int)(unit % a.nz) : 0;
   float2* s = lds + r * N;
   FftTw<N, E> twr;
-  fft_load_twiddles<N, E>(twr, t, a.twx);
+  fft_load_twiddles<N, E>(twr, t0, a.twx);
   const float2* tab = a.tabx + (int64_t)(j * 2 + 0) * 3 * (N / 2);
   const int64_t plane = (int64_t)a.nzpsi * a.npsi;
   for (int64_t b = 0; b < a.nang; ++b) {
+    // recompute thread-dependent addresses every angle instead of keeping
+    // them live across the loop (register pressure)
+    int t = t0;
+    asm volatile("" : "+v"(t));
     const int64_t bj = b * a.ndist + j;
     const float mb = a.mag[bj], ry = a.pos[2 * bj], rx = a.pos[2 * bj + 1];
     const float y = bs_coord(ty, a.nz, a.nzpsi, mb, ry);
@@ -140,12 +144,12 @@ __global__ __launch_bounds__(kRowThreads) void bh_fwd_rows_k(BhArgs a) {
     const TO* cb = (const TO*)a.c + b * plane;
     const int p = a.pidx ? a.pidx[b] : 0;
     const float2* prow = a.prb + (((int64_t)p * a.ndist + j) * a.nz + ty) * N;
-#pragma unroll
+#pragma unroll 1
     for (int e = 0; e < E; ++e) {
       const int i = t + e * T;
       TO v, d1, d2;
       s_gather<TO, false>(cb, a.npsi, a.nzpsi, bs_coord(i, N, a.npsi, mb, rx), iy, wy, wy, v, d1, d2);
-      s[dct_slot<N>(i)] = cmul(prow[i], expi(to_c(v)));
+      s[dct_slot<N>(i)] = cmul(prow[i], expi_fast(to_c(v)));
     }
     __syncthreads();
     fresnel_1d<N, E>(s, t, twr, tab);
@@ -179,21 +183,23 @@ __global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_f
   const float* dbase = a.data + f * (int64_t)N * a.n + x0;
   float dv[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) dv[e] = dbase[(int64_t)dct_nat<N>(t + e * T) * a.n + cc];
+  for (int e = 0; e < E; ++e) dv[e] = dbase[dct_nat<N>(t + e * T) * a.n + cc];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int idx = threadIdx.x + e * NT;
     const int c = idx % C, y = idx / C;
-    lds[c * LS + dct_slot<N>(y)] = base[(int64_t)y * a.n + c];
+    lds[c * LS + dct_slot<N>(y)] = base[y * a.n + c];
   }
   __syncthreads();
   float2* s = lds + cc * LS;
   fresnel_1d<N, E>(s, t, twr, a.taby + (int64_t)(j * 2 + 0) * 3 * (N / 2));
   const float sc = a.scales[b].x;
   double obj = 0.0;
+  int tr = t;
+  asm volatile("" : "+v"(tr));
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const int p = t + e * T;
+    const int p = tr + e * T;
     const float2 v = s[p];
     const float am = sqrtf(v.x * v.x + v.y * v.y);
     const float q = am - dv[e];
@@ -203,11 +209,13 @@ __global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_f
   }
   __syncthreads();
   fresnel_1d<N, E>(s, t, twr, a.taby + (int64_t)(j * 2 + 1) * 3 * (N / 2));
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const int idx = threadIdx.x + e * NT;
+    const int idx = tid + e * NT;
     const int c = idx % C, y = idx / C;
-    base[(int64_t)y * a.n + c] = lds[c * LS + dct_slot<N>(y)];
+    base[y * a.n + c] = lds[c * LS + dct_slot<N>(y)];
   }
   if constexpr (NT >= 64) {
 #pragma unroll
@@ -237,35 +245,37 @@ template <int N, int E, typename TO>
 __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
   constexpr int T = N / E, ROWS = kRowThreads / T;
   __shared__ float2 lds[ROWS * N];
+  __shared__ float2 lacc[ROWS * N];  // probe-gradient accumulator of the row
   __shared__ float2 red[kRowThreads / 64];
-  const int r = threadIdx.x / T, t = threadIdx.x % T;
+  const int r = threadIdx.x / T, t0 = threadIdx.x % T;
   const int64_t nunits = (int64_t)a.ndist * a.nz;
-  const int64_t unit = (int64_t)blockIdx.x * ROWS + r;
+  const int64_t unit = xcd_tile(blockIdx.x, gridDim.x) * ROWS + r;  // XCD-contiguous row bands
   const bool valid = unit < nunits;
   const int j = valid ? (int)(unit / a.nz) : 0, ty = valid ? (int)(unit % a.nz) : 0;
   float2* s = lds + r * N;
   FftTw<N, E> twr;
-  fft_load_twiddles<N, E>(twr, t, a.twx);
+  fft_load_twiddles<N, E>(twr, t0, a.twx);
   const float2* tab = a.tabx + (int64_t)(j * 2 + 1) * 3 * (N / 2);
   const int64_t plane = (int64_t)a.nzpsi * a.npsi;
-  float2 acc[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) acc[e] = make_float2(0.f, 0.f);
+  float2* acc = lacc + r * N;  // element i is only touched by thread t = i % T
+  for (int e = 0; e < E; ++e) acc[t0 + e * T] = make_float2(0.f, 0.f);
   int pcur = a.pidx ? a.pidx[0] : 0;
   auto flush = [&](int p) {
     if (!valid) return;
     float2* g = a.gprb + (((int64_t)p * a.ndist + j) * a.nz + ty) * N;
+    const int t = t0;
     const float rho = a.rho_prb[p];
-#pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = t + e * T;
-      float2 v = cscale(acc[e], rho);
+      float2 v = cscale(acc[i], rho);
       if (a.accumulate_prb) v = cadd(g[i], v);
       g[i] = v;
-      acc[e] = make_float2(0.f, 0.f);
+      acc[i] = make_float2(0.f, 0.f);
     }
   };
   for (int64_t b = 0; b < a.nang; ++b) {
+    int t = t0;
+    asm volatile("" : "+v"(t));
     const int64_t bj = b * a.ndist + j;
     float2* row = a.T + (bj * a.nz + ty) * (int64_t)N;
 #pragma unroll
@@ -290,14 +300,14 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
     const TO* cb = (const TO*)a.c + b * plane;
     const float2* prow = a.prb + (((int64_t)p * a.ndist + j) * a.nz + ty) * N;
     float s1 = 0.f, s2 = 0.f;
-#pragma unroll
+#pragma unroll 1
     for (int e = 0; e < E; ++e) {
       const int i = t + e * T;
       const float2 y12 = s[dct_slot<N>(i)];
       TO v, d1, d2;
       s_gather<TO, true>(cb, a.npsi, a.nzpsi, bs_coord(i, N, a.npsi, mb, rx), iy, wy, dwy, v, d1, d2);
-      const float2 ex = expi(to_c(v));
-      acc[e] = cadd(acc[e], cmulc(y12, ex));                // y12 conj(x12)
+      const float2 ex = expi_fast(to_c(v));
+      acc[i] = cadd(acc[i], cmulc(y12, ex));                // y12 conj(x12)
       const float2 q = cmulc(cmulc(y12, prow[i]), ex);      // y12 conj(prb) conj(x12)
       const TO g = from_c<TO>(make_float2(q.y, -q.x));      // * (-i)
       s1 += vredot(g, d1);
@@ -305,7 +315,7 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
       if (valid) row[i] = to_c(g);
     }
     const float2 tot = row_sum2<T>(make_float2(s1, s2), red);
-    if (valid && t == 0) a.part_r[bj * a.nz + ty] = tot;
+    if (valid && t0 == 0) a.part_r[bj * a.nz + ty] = tot;
     __syncthreads();
   }
   flush(pcur);
@@ -316,7 +326,7 @@ template <typename TO>
 __global__ __launch_bounds__(256) void bh_sadj_k(BhArgs a, TO* __restrict__ dpsi, int tiles_x,
                                                  int tiles_y) {
   constexpr int TX = 64, TY = 4;
-  const int64_t bid = blockIdx.x;
+  const int64_t bid = xcd_tile(blockIdx.x, gridDim.x);
   const int txt = (int)(bid % tiles_x);
   const int64_t rest = bid / tiles_x;
   const int tyt = (int)(rest % tiles_y);
@@ -397,9 +407,12 @@ struct BhLayout {
   int64_t c = 0, T = 0, dpsi = 0, part_r = 0, part_obj = 0, total = 0;
 };
 
+template <int N>
+constexpr int bh_col_e();
+
 template <int NZ>
 static int col_tiles_c(int64_t n) {
-  constexpr int TT = NZ / col_e<NZ>();
+  constexpr int TT = NZ / bh_col_e<NZ>();
   constexpr int CT = 256 / TT, CL = 8192 / NZ;
   constexpr int CM0 = CT < CL ? CT : CL;
   constexpr int CM = CM0 > 16 ? 16 : (CM0 < 1 ? 1 : CM0);
@@ -473,9 +486,16 @@ static hipError_t rows_dispatch(bool adjoint, const BhArgs& a, hipStream_t st) {
   }
 }
 
+// Column pass of the fused gradient: 16 elements per thread at N = 2048 (two
+// columns per 256-thread workgroup), else 8.
+template <int N>
+constexpr int bh_col_e() {
+  return N >= 2048 ? 16 : 8;
+}
+
 template <int NZ>
 static hipError_t launch_cols(const BhArgs& a, hipStream_t st) {
-  constexpr int E = col_e<NZ>();
+  constexpr int E = bh_col_e<NZ>();
   const int C = cols_per_tile(NZ, a.n);
   const int tpf = a.n / C;
   const int64_t grid = a.nang * a.ndist * tpf;
