@@ -1,0 +1,140 @@
+"""Serving loop: scheduling core on the CPU (fake executor), result identity
+alone vs in the mix and graph replay on the GPU."""
+import numpy as np
+import pytest
+
+from holotomocupy_amd.serve import BHServer, Geometry, GradientRequest, Scheduler
+
+
+class _T:
+    """Stand-in for a device tensor: only .shape is used by the scheduler."""
+    def __init__(self, *shape):
+        self.shape = shape
+
+
+class FakeExec:
+    def __init__(self, per_angle=100):
+        self.per_angle = per_angle
+        self.runs = []
+        self.pending_done = set()
+
+    def footprint(self, req):
+        return req.nangles * self.per_angle
+
+    def run(self, geo, batch):
+        h = len(self.runs)
+        self.runs.append((geo, [r.nangles for r in batch]))
+        return h
+
+    def done(self, h):
+        return h in self.pending_done
+
+    def finish(self, h, batch):
+        for r in batch:
+            r.future.set_result(h)
+
+
+def req(geo, a):
+    return GradientRequest(geo, _T(1), _T(a, 4, 4), _T(a, 1, 2), _T(a, 1, 4, 4), data_size=1.0)
+
+
+G1 = Geometry(8, 8, 8, 8, 1)
+G2 = Geometry(16, 16, 16, 16, 1)
+
+
+def test_admission_by_budget_and_release():
+    ex = FakeExec()
+    s = Scheduler(ex, hbm_budget=3000, max_batch_angles=64, max_inflight=1)
+    rs = [req(G1, 16) for _ in range(3)]  # 1600 bytes each
+    for r in rs:
+        s.submit(r)
+    s.step()
+    assert s.in_use == 1600 and len(s.pending) == 2 and ex.runs == [(G1, [16])]
+    ex.pending_done.add(0)
+    s.step()
+    assert rs[0].future.result() == 0
+    assert len(ex.runs) == 2 and len(s.pending) == 1
+    ex.pending_done.update({1, 2})
+    for _ in range(3):
+        s.step()
+    assert all(r.future.done() for r in rs) and s.idle() and s.in_use == 0
+
+
+def test_rejects_oversize_requests():
+    ex = FakeExec()
+    s = Scheduler(ex, hbm_budget=1000, max_batch_angles=8)
+    with pytest.raises(MemoryError):
+        s.submit(req(G1, 16)).result()
+    with pytest.raises(ValueError):
+        Scheduler(ex, 10 ** 9, max_batch_angles=8).submit(req(G1, 16)).result()
+
+
+def test_batches_per_geometry_fifo_and_fair():
+    ex = FakeExec(per_angle=1)
+    s = Scheduler(ex, hbm_budget=10 ** 9, max_batch_angles=64, max_inflight=4)
+    order = [(G1, 32), (G2, 16), (G1, 32), (G1, 16), (G2, 64), (G1, 8)]
+    for g, a in order:
+        s.submit(req(g, a))
+    s.step()
+    # first bucket: G1 gets 32 + 32 (16 would exceed 64); then G2: 16 (64 would exceed); buckets alternate
+    assert ex.runs[0] == (G1, [32, 32])
+    assert ex.runs[1] == (G2, [16])
+    assert ex.runs[2] == (G1, [16, 8])
+    assert ex.runs[3] == (G2, [64])
+
+
+def test_server_thread_with_fake_executor():
+    ex = FakeExec()
+    srv = BHServer(hbm_budget=10 ** 9, executor=ex)
+    futs = [srv.submit(req(G1, 8)) for _ in range(5)]
+    import time
+    t0 = time.time()
+    while len(ex.runs) < 1 and time.time() - t0 < 5:
+        time.sleep(0.001)
+    ex.pending_done.update(range(100))
+    assert all(f.result(timeout=5) is not None for f in futs)
+    srv.close()
+
+
+# ------------------------------------------------------------------ GPU
+def _gpu_request(rng, geo, a, data_size, rho, scale):
+    import torch
+    from holotomocupy_amd import phantom
+    prb = torch.from_numpy(phantom.probe(rng, geo.ndist, geo.nz, geo.n) * scale).cuda()
+    proj = torch.from_numpy(phantom.projections(rng, a, geo.nzobj, geo.nobj)).cuda()
+    pos = torch.from_numpy(phantom.positions(rng, a, geo.ndist, 2.0)).cuda()
+    data = torch.from_numpy(rng.uniform(0.5, 1.5, (a, geo.ndist, geo.nz, geo.n)).astype(np.float32)).cuda()
+    return GradientRequest(geo, prb, proj, pos, data, data_size=data_size, rho=rho)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("graphs", [False, True])
+def test_result_same_alone_and_in_mix(gpu, graphs):
+    import torch
+    print(f"[case] serve alone vs mix graphs={graphs}", flush=True)
+    rng = np.random.default_rng(77)
+    ga = Geometry(64, 64, 48, 48, 3, detector_pixelsize=1.476e-6 * 64)
+    gb = Geometry(32, 32, 40, 40, 2, detector_pixelsize=1.476e-6 * 128)
+    specs = [(ga, 4, 1e6, (1, 0.5, 0.2), 1.0), (gb, 3, 2e5, (1, 1, 1), 0.7), (ga, 5, 3e6, (2, 0.1, 0.3), 1.3),
+             (ga, 2, 1e6, (1, 1, 1), 0.9)]
+    reqs = [_gpu_request(np.random.default_rng(100 + i), *sp) for i, sp in enumerate(specs)]
+    srv = BHServer(hbm_budget=8 << 30, use_graphs=graphs)
+    alone = []
+    for r in reqs:
+        alone.append([t.clone() for t in srv.submit(GradientRequest(r.geometry, r.prb, r.proj, r.pos, r.data,
+                                                                     r.data_size, r.rho)).result(timeout=60)])
+    mixed = [srv.submit(GradientRequest(r.geometry, r.prb, r.proj, r.pos, r.data, r.data_size, r.rho))
+             for r in reqs]
+    for _ in range(3):  # replays with eager work in between
+        x = torch.randn(1000, device="cuda")
+        torch.cuda.synchronize()
+        again = [srv.submit(GradientRequest(r.geometry, r.prb, r.proj, r.pos, r.data, r.data_size, r.rho))
+                 for r in reqs]
+        for a, f in zip(alone, again):
+            for u, v in zip(a, f.result(timeout=60)):
+                assert torch.equal(u, v)
+        del x
+    for a, f in zip(alone, mixed):
+        for u, v in zip(a, f.result(timeout=60)):
+            assert torch.equal(u, v)
+    srv.close()
