@@ -35,6 +35,10 @@ def parse():
     p.add_argument("--ndist", type=int, default=4)
     p.add_argument("--path", choices=["auto", "plain", "fused", "served"], default="auto")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison run")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="first run a 256 MiB device-to-device copy 3 times (a known-size kernel for "
+                        "scaling FETCH_SIZE / WRITE_SIZE counters)")
     return p.parse_args()
 
 
@@ -78,6 +82,35 @@ def plain_step(rec, vars_, grads, data):
 def fused_step(rec, vars_, grads, data):
     grads["prb"].zero_()
     rec.gradients_fused(vars_, grads, data)
+
+
+def served_run(a, rec, vars_, data, nreq, warmup):
+    """Submit nreq gradient requests of A angles (the request's own probe,
+    projections, positions and data) to the serving loop as one burst; returns
+    wall time and per-request latencies (ms).  `warmup` identical bursts run
+    first, untimed, so plans, staging and the graphs of every batch shape on
+    both streams exist before the timed burst."""
+    import torch
+    from holotomocupy_amd.serve import BHServer, Geometry, GradientRequest
+    g = geometry(a.n, a.ndist)
+    geo = Geometry(a.n, a.n, a.nobj, a.nobj, a.ndist, energy=g["energy"], z1=tuple(float(z) for z in g["z1"]),
+                   focustodetectordistance=g["focustodetectordistance"], detector_pixelsize=g["detector_pixelsize"])
+    srv = BHServer(max_batch_angles=max(a.angles, 64))
+    mk = lambda: GradientRequest(geo, vars_["prb"], vars_["proj"], vars_["pos"], data,  # noqa: E731
+                                 data_size=rec.data_size, rho=(1.0, 0.05, 0.02))
+    for _ in range(max(2, warmup)):
+        for f in [srv.submit(mk()) for _ in range(nreq)]:
+            f.result()
+    torch.cuda.synchronize()
+    reqs = [mk() for _ in range(nreq)]
+    t0 = time.perf_counter()
+    futs = [srv.submit(r) for r in reqs]
+    for f in futs:
+        f.result()
+    wall = time.perf_counter() - t0
+    srv.close()
+    lat = sorted((r.t_done - r.t_submit) * 1e3 for r in reqs)
+    return wall, lat
 
 
 def time_steps(fn, steps, warmup):
@@ -126,6 +159,66 @@ def roofline_fresnel_cols(rec, a, reps=20):
             "bytes_per_launch": nbytes, "avg_us": round(us, 2)}
 
 
+STAGES = ["prefilter", "rows_fwd", "cols", "rows_adj", "sadj", "prefilter_grad", "reductions"]
+
+# HBM bytes per launch of the fused passes at the default workload (2048^2
+# detector, 4 distances, 1024^2 object plane, 16 angles), from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE runs scaled by what the counters read for a 256 MiB
+# device-to-device copy (reads x2.00, writes x1.00); the commands and per-kernel
+# figures are in profiles/bh_gradient_fused_2048_d4_a16_traffic.md.
+TRAFFIC_MIB = {"bh_fwd_rows_k": 1709.5 + 2048.0, "bh_cols_k": 3100.1 + 2643.5, "bh_adj_rows_k": 3982.4 + 2177.2}
+TRAFFIC_WORKLOAD = (2048, 4, 1024, 16)
+
+
+def roofline_fused(a, rec, vars_, grads, data, steps=3):
+    """Per-pass durations of the fused gradient, measured with HIP events
+    recorded by the library around each pass on the launch stream, over
+    `steps` gradient evaluations of the same workload.  The dominant pass is
+    reported against its compulsory HBM bytes:
+      rows_fwd: A nd F + A Fo + nd F      (write work field; read c, probe)
+      cols:     2.5 A nd F                (read/write work field, read sqrt(data))
+      rows_adj: 2 A nd F + A Fo + 2 nd F  (read/write work field; read c, probe; write probe gradient)
+    with F = 8 nz n, Fo = 8 nzobj nobj bytes."""
+    import ctypes
+    import torch
+    from holotomocupy_amd import _lib
+    fused_step(rec, vars_, grads, data)  # plan + workspace
+    bh = rec._bh
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(2 * len(STAGES))]
+    for e in evs:  # torch creates the HIP event on first record
+        e.record()
+    torch.cuda.synchronize()
+    arr = (ctypes.c_void_p * len(evs))(*[ctypes.c_void_p(e.cuda_event) for e in evs])
+    if not all(x for x in arr):
+        raise RuntimeError("stage events were not created")
+    _lib.call("htamd_bh_set_stage_events", bh._plan, arr, len(evs))
+    tot = [0.0] * len(STAGES)
+    try:
+        for _ in range(steps):
+            fused_step(rec, vars_, grads, data)
+            torch.cuda.synchronize()
+            for k in range(len(STAGES)):
+                tot[k] += evs[2 * k].elapsed_time(evs[2 * k + 1]) / steps
+    finally:
+        _lib.call("htamd_bh_set_stage_events", bh._plan, None, 0)
+    A, nd = a.angles, a.ndist
+    F, Fo = 8 * a.n * a.n, 8 * a.nobj * a.nobj
+    model = {"rows_fwd": A * nd * F + A * Fo + nd * F, "cols": 2.5 * A * nd * F,
+             "rows_adj": 2 * A * nd * F + A * Fo + 2 * nd * F}
+    k = max(model, key=lambda s: tot[STAGES.index(s)])
+    us = tot[STAGES.index(k)] * 1e3
+    achieved = model[k] / (us * 1e-6) / 1e9
+    kname = {"rows_fwd": "bh_fwd_rows_k", "cols": "bh_cols_k", "rows_adj": "bh_adj_rows_k"}[k]
+    traffic = None
+    if (a.n, a.ndist, a.nobj, a.angles) == TRAFFIC_WORKLOAD:
+        traffic = int(TRAFFIC_MIB[kname] * 2 ** 20)
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": kname,
+            "bytes_per_launch": int(model[k]), "avg_us": round(us, 2),
+            "pass_ms": {s: round(t, 3) for s, t in zip(STAGES, tot)}}
+
+
 def cpu_baseline(a, host):
     """Oracle (NumPy/SciPy port of the reference cascade) on one angle of the
     same workload: one BH gradient evaluation, timed once."""
@@ -165,32 +258,44 @@ def main():
     rec, vars_, data, host = make_inputs(a)
     grads = {"prb": torch.zeros_like(vars_["prb"]), "proj": torch.empty_like(vars_["proj"]),
              "pos": torch.empty_like(vars_["pos"])}
-    path = "fused" if a.path == "auto" else a.path
-    if path == "served":
-        raise SystemExit("served path not built yet")
+    path = "served" if a.path == "auto" else a.path
+    if a.copy_calibration:
+        src = torch.ones(1 << 25, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     fn_plain = lambda: plain_step(rec, vars_, grads, data)  # noqa: E731
     if path == "plain":
         wall, per = time_steps(fn_plain, a.steps, a.warmup)
         plain_value = value = a.angles * a.steps / wall
     else:
-        n_plain = max(2, a.steps // 2)
-        wall_p, _ = time_steps(fn_plain, n_plain, 1)
-        plain_value = a.angles * n_plain / wall_p
-        fn = lambda: fused_step(rec, vars_, grads, data)  # noqa: E731
-        wall, per = time_steps(fn, a.steps, a.warmup)
+        plain_value = None
+        if not a.no_plain:
+            n_plain = max(2, a.steps // 2)
+            wall_p, _ = time_steps(fn_plain, n_plain, 1)
+            plain_value = a.angles * n_plain / wall_p
+        if path == "fused":
+            fn = lambda: fused_step(rec, vars_, grads, data)  # noqa: E731
+            wall, per = time_steps(fn, a.steps, a.warmup)
+        else:
+            wall, per = served_run(a, rec, vars_, data, a.steps, a.warmup)
         value = a.angles * a.steps / wall
     ms = wall * 1e3 / a.steps
-    roof = roofline_fresnel_cols(rec, a)
+    roof = roofline_fresnel_cols(rec, a) if path == "plain" else roofline_fused(a, rec, vars_, grads, data)
     out = {
         "metric": "BH gradient evals/s at 2048^2 (served vs plain), % HBM roofline, p50/p99 latency",
         "value": round(value, 3), "unit": "evals/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "c64", "data": "synthetic (seeded phantom projections, Gaussian probe, noisy amplitudes)",
         "config": {"workload": f"BH gradient (gradients_cascade), detector {a.n}x{a.n}, {a.ndist} distances, "
-                               f"object plane {a.nobj}x{a.nobj}, {a.angles} angles per request",
+                               f"object plane {a.nobj}x{a.nobj}, {a.angles} angles per request"
+                               + (f", {a.steps} requests through the serving loop" if path == "served" else ""),
                    "path": path, "angles_per_step": a.angles, "n": a.n, "ndist": a.ndist, "nobj": a.nobj},
         "latency_ms": {"p50": round(per[len(per) // 2], 3), "p99": round(per[min(len(per) - 1, int(0.99 * len(per)))], 3)},
-        "plain_value": round(plain_value, 3), "speedup_vs_plain": round(value / plain_value, 3),
+        "plain_value": round(plain_value, 3) if plain_value else None,
+        "speedup_vs_plain": round(value / plain_value, 3) if plain_value else None,
         "roofline": roof,
     }
     if not a.no_cpu_baseline:

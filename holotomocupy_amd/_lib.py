@@ -34,7 +34,13 @@ SIGNATURES = {
     "htamd_bh_plan_create": [c_vp, c_vp, ctypes.POINTER(c_vp)],
     "htamd_bh_plan_destroy": [c_vp],
     "htamd_bh_workspace_size": [c_vp, c_i64, ctypes.POINTER(c_i64)],
+    "htamd_bh_set_stage_events": [c_vp, ctypes.POINTER(c_vp), c_int],
     "htamd_bh_gradient": [c_vp, c_i64] + [c_vp] * 8 + [c_int] + [c_vp] * 5 + [c_i64, c_vp],
+    "htamd_tomo_plan_create": [c_i64, c_i64, c_vp, c_vp, ctypes.POINTER(c_vp)],
+    "htamd_tomo_plan_destroy": [c_vp],
+    "htamd_tomo_workspace_size": [c_vp, c_i64, ctypes.POINTER(c_i64)],
+    "htamd_tomo_R": [c_vp, c_vp, c_int, c_vp, c_i64, c_vp, c_i64, c_vp],
+    "htamd_tomo_RT": [c_vp, c_vp, c_int, c_vp, c_i64, c_vp, c_i64, c_vp],
     "htamd_shift_plan_create": [c_i64, c_i64, c_i64, c_i64, c_int, ctypes.POINTER(c_vp)],
     "htamd_shift_plan_destroy": [c_vp],
     "htamd_bspline_prefilter": [c_vp, c_vp, c_vp, c_i64, c_vp],

@@ -528,24 +528,49 @@ static hipError_t cols_dispatch(const BhArgs& a, hipStream_t st) {
   }
 }
 
+// Optional timing hook: events[2k] / events[2k+1] are recorded before / after
+// stage k (0 prefilter, 1 rows forward, 2 columns, 3 rows adjoint, 4 Sadj,
+// 5 prefilter of the gradient, 6 reductions) on the launch stream.
+struct StageEvents {
+  hipEvent_t* ev = nullptr;
+  int count = 0;
+  void mark(int idx, hipStream_t st) const {
+    if (ev && idx < count && ev[idx]) (void)hipEventRecord(ev[idx], st);
+  }
+};
+
 template <typename TO>
 static hipError_t bh_run(const htamd_shift_plan* sp, BhArgs a, const void* proj, void* gproj,
-                         float* gpos, double* objective, const BhLayout& L, char* ws, hipStream_t st) {
+                         float* gpos, double* objective, const BhLayout& L, char* ws, hipStream_t st,
+                         const StageEvents& sev) {
   hipError_t e;
   // coefficients of the projections (coeff, shift.py:74-82)
+  sev.mark(0, st);
   if ((e = bs_launch_prefilter<TO>((const TO*)proj, (TO*)(ws + L.c), a.nang, a.nzpsi, a.npsi, sp->taps,
                                    st)) != hipSuccess)
     return e;
+  sev.mark(1, st);
+  sev.mark(2, st);
   if ((e = rows_dispatch<TO>(false, a, st)) != hipSuccess) return e;
+  sev.mark(3, st);
+  sev.mark(4, st);
   if ((e = cols_dispatch(a, st)) != hipSuccess) return e;
+  sev.mark(5, st);
+  sev.mark(6, st);
   if ((e = rows_dispatch<TO>(true, a, st)) != hipSuccess) return e;
+  sev.mark(7, st);
   constexpr int TX = 64, TY = 4;
   const int tx = (a.npsi + TX - 1) / TX, ty = (a.nzpsi + TY - 1) / TY;
   TO* dpsi = (TO*)(ws + L.dpsi);
+  sev.mark(8, st);
   hipLaunchKernelGGL((bh_sadj_k<TO>), dim3((unsigned)(a.nang * tx * ty)), dim3(256), 0, st, a, dpsi, tx, ty);
   if ((e = hipGetLastError()) != hipSuccess) return e;
+  sev.mark(9, st);
+  sev.mark(10, st);
   if ((e = bs_launch_prefilter<TO>(dpsi, (TO*)gproj, a.nang, a.nzpsi, a.npsi, sp->taps, st)) != hipSuccess)
     return e;
+  sev.mark(11, st);
+  sev.mark(12, st);
   hipLaunchKernelGGL(bh_pos_final_k, dim3((unsigned)(a.nang * a.ndist)), dim3(256), 0, st, a.part_r, a.nz,
                      a.ndist, a.scales, gpos);
   if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -554,6 +579,7 @@ static hipError_t bh_run(const htamd_shift_plan* sp, BhArgs a, const void* proj,
     hipLaunchKernelGGL(bh_sum_k, dim3(1), dim3(256), 0, st, a.part_obj, ntiles, 1.0, objective);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
+  sev.mark(13, st);
   return hipSuccess;
 }
 
@@ -564,6 +590,7 @@ using namespace htamd;
 struct htamd_bh_plan {
   const htamd_prop_plan* prop = nullptr;
   const htamd_shift_plan* shift = nullptr;
+  StageEvents events;
 };
 
 extern "C" {
@@ -585,6 +612,13 @@ int htamd_bh_plan_create(const htamd_prop_plan* prop, const htamd_shift_plan* sh
 
 int htamd_bh_plan_destroy(htamd_bh_plan* p) {
   delete p;
+  return HTAMD_OK;
+}
+
+int htamd_bh_set_stage_events(htamd_bh_plan* p, void** events, int count) {
+  if (!p || count < 0 || (count > 0 && !events)) return fail(HTAMD_ERR_ARG, "bh_set_stage_events: bad arguments");
+  p->events.ev = (hipEvent_t*)events;
+  p->events.count = count;
   return HTAMD_OK;
 }
 
@@ -636,8 +670,8 @@ int htamd_bh_gradient(const htamd_bh_plan* p, int64_t nangles, const void* proj,
   a.tabx = p->prop->tab_x;
   a.taby = p->prop->tab_y;
   hipError_t e = p->shift->is_complex
-      ? bh_run<float2>(p->shift, a, proj, gproj, gpos, objective, L, ws, (hipStream_t)stream)
-      : bh_run<float>(p->shift, a, proj, gproj, gpos, objective, L, ws, (hipStream_t)stream);
+      ? bh_run<float2>(p->shift, a, proj, gproj, gpos, objective, L, ws, (hipStream_t)stream, p->events)
+      : bh_run<float>(p->shift, a, proj, gproj, gpos, objective, L, ws, (hipStream_t)stream, p->events);
   return e == hipSuccess ? HTAMD_OK : hip_fail(e, "bh_gradient");
 }
 

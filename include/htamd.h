@@ -193,6 +193,13 @@ typedef struct htamd_bh_plan htamd_bh_plan;
 int htamd_bh_plan_create(const htamd_prop_plan* prop, const htamd_shift_plan* shift, htamd_bh_plan** plan);
 int htamd_bh_plan_destroy(htamd_bh_plan* plan);
 int htamd_bh_workspace_size(const htamd_bh_plan* plan, int64_t nangles, int64_t* bytes);
+/* Profiling hook (no counterpart in the reference): events[2k] and
+ * events[2k+1] (hipEvent_t, caller-created; NULL entries skipped) are recorded
+ * on the launch stream before and after stage k of every later
+ * htamd_bh_gradient call on this plan: 0 prefilter, 1 rows forward, 2 columns,
+ * 3 rows adjoint, 4 Sadj, 5 prefilter of the object gradient, 6 reductions.
+ * count = 0 switches it off.  Not for concurrent use of one plan. */
+int htamd_bh_set_stage_events(htamd_bh_plan* plan, void** events, int count);
 /* Replaces Rec.gradients_cascade / _gradients_cascade and the gF0..gF3 chain
  * (src/holotomocupy/rec_mpi.py:327-353, 555-813). */
 int htamd_bh_gradient(const htamd_bh_plan* plan, int64_t nangles, const void* proj, const float* pos,
@@ -200,6 +207,31 @@ int htamd_bh_gradient(const htamd_bh_plan* plan, int64_t nangles, const void* pr
                       const float* angle_scales, const float* rho_prb, int accumulate_prb, void* gproj,
                       float* gpos, void* gprb, double* objective, void* workspace,
                       int64_t workspace_bytes, void* stream);
+
+/* ---- USFFT Radon transform ---------------------------------------------------
+ * Geometry of Tomo(n, nz, theta, mask_r) (src/holotomocupy/tomo.py:12-54): slices
+ * n x n, n a power of two in [8, 1024], ntheta angles (radians; rounded to
+ * float32 as the reference stores them), mask [n, n] float64 support mask
+ * (NULL = 1; the smooth circular mask of tomo.py:34-44 is built by the caller).
+ * eps = 1e-3, mu and the gridding half-width m follow tomo.py:14-16. */
+typedef struct htamd_tomo_plan htamd_tomo_plan;
+int htamd_tomo_plan_create(int64_t n, int64_t ntheta, const double* theta, const double* mask,
+                           htamd_tomo_plan** plan);
+int htamd_tomo_plan_destroy(htamd_tomo_plan* plan);
+/* Device scratch for nz slices: nz * 6 n^2 complex64 (the 2n x 2n frequency
+ * grid plus one half-transformed n x 2n stage). */
+int htamd_tomo_workspace_size(const htamd_tomo_plan* plan, int64_t nz, int64_t* bytes);
+/* Replaces Tomo.R (src/holotomocupy/tomo.py:56-86) with the gather kernel
+ * (src/holotomocupy/cuda_kernels.py:3-76, dir 0): obj [nz, n, n] -> out
+ * [ntheta, nz, n], float32 (real part kept, as for a float32 object) or
+ * complex64 when is_complex. */
+int htamd_tomo_R(const htamd_tomo_plan* plan, const void* obj, int is_complex, void* out, int64_t nz,
+                 void* workspace, int64_t workspace_bytes, void* stream);
+/* Replaces Tomo.RT (src/holotomocupy/tomo.py:88-116) with the scatter of
+ * cuda_kernels.py:3-76 (dir 1, float atomics): data [ntheta, nz, n] ->
+ * out [nz, n, n]; dtypes as for htamd_tomo_R. */
+int htamd_tomo_RT(const htamd_tomo_plan* plan, const void* data, int is_complex, void* out, int64_t nz,
+                  void* workspace, int64_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,142 @@
+"""GPU parity of the USFFT Radon transform R / RT against the FP64 oracle.
+
+Tolerance: relative L2 <= 1e-5 (FP32 kernels vs FP64 oracle; the angles are
+rounded to float32 on both sides, as the reference stores them).  At the
+bench size (256^3 slab share, 384 angles) the checks are size-independent:
+adjointness of RT to R, linearity, and the projection trajectory of an
+off-centre blob.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import tomo as ot
+
+pytestmark = pytest.mark.gpu
+REL_TOL = 1e-5
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - b) / np.linalg.norm(b))
+
+
+def dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def data(rng, shape, real):
+    x = rng.standard_normal(shape)
+    if not real:
+        x = x + 1j * rng.standard_normal(shape)
+    return x.astype(np.float32 if real else np.complex64)
+
+
+def angles(ntheta, span=np.pi):
+    return (np.arange(ntheta) * span / ntheta).astype(np.float32).astype(np.float64)
+
+
+CASES = [  # n, nz, ntheta, mask_r, nz used
+    (8, 2, 5, 0, 2),
+    (16, 3, 12, 0, 3),
+    (32, 2, 24, 0.9, 2),
+    (64, 4, 48, 0, 3),
+    (128, 2, 30, 0.8, 2),
+]
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("real", [True, False])
+def test_R_matches_oracle(gpu, case, real):
+    from holotomocupy_amd import Tomo
+    n, nz, ntheta, mask_r, nzu = case
+    print(f"[case] tomo R n={n} nz={nzu} ntheta={ntheta} real={real}", flush=True)
+    rng = np.random.default_rng(n + ntheta)
+    th = angles(ntheta)
+    g, o = Tomo(n, nz, th, mask_r), ot.Tomo(n, nz, th, mask_r)
+    x = data(rng, (nzu, n, n), real)
+    y = g.R(dev(x))
+    torch.cuda.synchronize()
+    assert y.dtype == (torch.float32 if real else torch.complex64) and tuple(y.shape) == (ntheta, nzu, n)
+    assert rel(y.cpu().numpy(), o.R(x.astype(np.float64 if real else np.complex128))) < REL_TOL
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("real", [True, False])
+def test_RT_matches_oracle(gpu, case, real):
+    from holotomocupy_amd import Tomo
+    n, nz, ntheta, mask_r, nzu = case
+    print(f"[case] tomo RT n={n} nz={nzu} ntheta={ntheta} real={real}", flush=True)
+    rng = np.random.default_rng(7 * n + ntheta)
+    th = angles(ntheta)
+    g, o = Tomo(n, nz, th, mask_r), ot.Tomo(n, nz, th, mask_r)
+    d = data(rng, (ntheta, nzu, n), real)
+    x = g.RT(dev(d))
+    torch.cuda.synchronize()
+    assert tuple(x.shape) == (nzu, n, n)
+    assert rel(x.cpu().numpy(), o.RT(d.astype(np.float64 if real else np.complex128))) < REL_TOL
+
+
+def test_nonuniform_angles_and_full_circle(gpu):
+    from holotomocupy_amd import Tomo
+    print("[case] tomo random angles n=32", flush=True)
+    rng = np.random.default_rng(3)
+    th = np.sort(rng.uniform(-np.pi, 2 * np.pi, 17)).astype(np.float32).astype(np.float64)
+    g, o = Tomo(32, 1, th, 0), ot.Tomo(32, 1, th, 0)
+    x = data(rng, (1, 32, 32), False)
+    assert rel(g.R(dev(x)).cpu().numpy(), o.R(x.astype(np.complex128))) < REL_TOL
+    d = data(rng, (17, 1, 32), False)
+    assert rel(g.RT(dev(d)).cpu().numpy(), o.RT(d.astype(np.complex128))) < REL_TOL
+
+
+def test_bench_size_adjoint_linearity_trajectory(gpu):
+    """256^3 share with 384 angles: <R x, y> = <x, RT y>, R linear, blob on
+    its sinusoid (size-independent properties; the oracle is too slow here)."""
+    from holotomocupy_amd import Tomo
+    n, nz, ntheta = 256, 4, 384
+    print(f"[case] tomo bench-size n={n} nz={nz} ntheta={ntheta}", flush=True)
+    th = angles(ntheta)
+    g = Tomo(n, nz, th, 0)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    x = torch.randn((nz, n, n), dtype=torch.complex64, device="cuda", generator=gen)
+    x2 = torch.randn((nz, n, n), dtype=torch.complex64, device="cuda", generator=gen)
+    y = torch.randn((ntheta, nz, n), dtype=torch.complex64, device="cuda", generator=gen)
+    Rx, RTy = g.R(x), g.RT(y)
+    lhs = torch.vdot(Rx.flatten().to(torch.complex128), y.flatten().to(torch.complex128))
+    rhs = torch.vdot(x.flatten().to(torch.complex128), RTy.flatten().to(torch.complex128))
+    assert abs(complex(lhs - rhs)) < 1e-5 * abs(complex(lhs))
+    lin = g.R(2.0 * x - 0.5j * x2) - (2.0 * Rx - 0.5j * g.R(x2))
+    assert float(torch.linalg.norm(lin) / torch.linalg.norm(Rx)) < 1e-5
+    i, j = np.mgrid[0:n, 0:n]
+    ic, jc = n / 2 + 40.0, n / 2 - 25.0
+    blob = np.exp(-((i - ic) ** 2 + (j - jc) ** 2) / (2 * 3.0 ** 2)).astype(np.float32)
+    p = g.R(dev(blob[None].repeat(nz, 0))).cpu().numpy()[:, 0].astype(np.float64)
+    w = np.abs(p) ** 4
+    t_peak = (w * np.arange(n)).sum(1) / w.sum(1)
+    t_ref = n / 2 + (jc - n / 2) * np.cos(th) - (ic - n / 2) * np.sin(th)
+    assert np.abs(t_peak - t_ref).max() < 0.2
+
+
+def test_rejects_bad_sizes(gpu):
+    from holotomocupy_amd import Tomo
+    from holotomocupy_amd._lib import HtamdError
+    with pytest.raises(HtamdError):
+        Tomo(48, 1, angles(4), 0)
+    g = Tomo(16, 2, angles(4), 0)
+    with pytest.raises(ValueError):
+        g.R(torch.zeros((3, 16, 16), device="cuda"))
+
+
+def test_fbp_and_rec_tomo_shapes(gpu):
+    from holotomocupy_amd import Tomo
+    print("[case] tomo fbp / rec_tomo n=32", flush=True)
+    n, ntheta = 32, 48
+    th = angles(ntheta)
+    g = Tomo(n, 1, th, 0)
+    i, j = np.mgrid[0:n, 0:n]
+    f = np.exp(-((i - n / 2 - 3) ** 2 + (j - n / 2 + 4) ** 2) / (2 * 3.0 ** 2)).astype(np.float32)[None]
+    d = g.R(dev(f))
+    rec = g.fbp(d, "shepp")
+    assert tuple(rec.shape) == (1, n, n) and rec.dtype == torch.float32
+    u = g.rec_tomo(d, 8)
+    r0 = float(torch.linalg.norm(d))
+    assert float(torch.linalg.norm(g.R(u) - d)) < 0.2 * r0
