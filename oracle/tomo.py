@@ -15,8 +15,9 @@ x0 = (t - n/2) cos(theta)/n, y0 = -(t - n/2) sin(theta)/n.
 Pins (tests/test_oracle_tomo.py): RT is the exact adjoint of R; an off-centre
 Gaussian blob projects onto the trajectory t = n/2 + n (x_c cos - y_c sin)
 (sign of cuda_kernels.py:22-23) and a centred one gives angle-independent
-profiles; R of a smooth object agrees with a direct FP64 line integral up to
-the USFFT accuracy (eps = 1e-3) after one global scale.
+profiles; R of a smooth object equals its direct FP64 line integrals (in
+pixels) divided by sqrt(n ntheta), to the USFFT accuracy (eps = 1e-3), which
+pins the absolute normalisation.
 """
 import math
 

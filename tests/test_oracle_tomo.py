@@ -49,11 +49,13 @@ def test_blob_trajectory_and_centre_invariance():
     assert np.abs(c - c[0]).max() < 2e-3 * np.abs(c).max()
 
 
-def test_matches_line_integrals_up_to_scale():
+@pytest.mark.parametrize("n,ntheta", [(64, 16), (64, 32), (128, 16)])
+def test_matches_line_integrals_with_absolute_scale(n, ntheta):
     """Direct FP64 line integrals p(theta, s) = int f(x, y) along
-    x cos - y sin = s, against R after one global least-squares scale."""
-    n = 64
-    theta = np.arange(16) * np.pi / 16
+    x cos - y sin = s (in pixels), against R: R = p / sqrt(n ntheta), the
+    normalisation fixed by phi's 1/(n sqrt(n ntheta)) and the 1/4, 1/n of the
+    gather and line transform (a dropped or doubled factor fails)."""
+    theta = np.arange(ntheta) * np.pi / ntheta
     T = ot.Tomo(n, 1, theta, 0)
     f = blob(n, n / 2 + 5, n / 2 - 3, 4.0) + 0.5 * blob(n, n / 2 - 8, n / 2 + 6, 3.0)
     p = T.R(f[None])[:, 0].real
@@ -66,5 +68,7 @@ def test_matches_line_integrals_up_to_scale():
         ys = -s[:, None] * np.sin(th) + u[None, :] * np.cos(th)
         vals = ndi.map_coordinates(f, [ys + n / 2, xs + n / 2], order=3, mode="constant")
         ref[a] = np.trapezoid(vals, u, axis=1)
+    ref /= np.sqrt(n * ntheta)
+    assert np.linalg.norm(p - ref) / np.linalg.norm(p) < 5e-3
     scale = np.sum(p * ref) / np.sum(ref * ref)
-    assert np.linalg.norm(p - scale * ref) / np.linalg.norm(p) < 5e-3
+    assert abs(scale - 1) < 1e-4
