@@ -87,6 +87,69 @@ __device__ __forceinline__ void s_gather(const TO* __restrict__ c, int npsi, int
   }
 }
 
+// Row-staged gather.  All pixels of a detector row sample the same four
+// object rows (iy .. iy+3), so those rows are combined along y once per
+// (angle, row) into LDS with coalesced reads of c, and each pixel then reads 4
+// LDS values instead of 16 scattered global ones.  cy[x - lo] = sum_jy wy c,
+// cdy[x - lo] = sum_jy dwy c over the columns [lo, hi) the row's taps reach
+// (the same sums as s_gather, combined along y first).
+__device__ __forceinline__ unsigned char* dyn_lds() {
+  extern __shared__ __align__(16) unsigned char htamd_dyn_lds[];
+  return htamd_dyn_lds;
+}
+
+template <int N>
+__device__ __forceinline__ void row_span(int npsi, float mb, float rx, int& lo, int& hi) {
+  // bs_coord is monotone in the pixel index, so the row's taps lie within
+  // [floor(x(0)) - 1, floor(x(N-1)) + 2]
+  const float x0 = bs_coord(0, N, npsi, mb, rx), x1 = bs_coord(N - 1, N, npsi, mb, rx);
+  lo = max((int)floorf(fminf(x0, x1)) - 1, 0);
+  hi = min((int)floorf(fmaxf(x0, x1)) + 3, npsi);
+}
+
+template <typename TO, int T, bool DERIV>
+__device__ __forceinline__ void stage_row(const TO* __restrict__ cb, int npsi, int nzpsi, int iy,
+                                          const float* wy, const float* dwy, int lo, int hi, int t,
+                                          TO* cy, TO* cdy) {
+  for (int x = lo + t; x < hi; x += T) {
+    TO acc = vzero<TO>(), dacc = vzero<TO>();
+#pragma unroll
+    for (int jy = 0; jy < 4; ++jy) {
+      const int yy = iy + jy;
+      if (yy < 0 || yy >= nzpsi) continue;
+      const TO cv = cb[(int64_t)yy * npsi + x];
+      acc = vfma(wy[jy], cv, acc);
+      if (DERIV) dacc = vfma(dwy[jy], cv, dacc);
+    }
+    cy[x - lo] = acc;
+    if (DERIV) cdy[x - lo] = dacc;
+  }
+}
+
+template <typename TO, bool DERIV>
+__device__ __forceinline__ void row_gather(const TO* cy, const TO* cdy, int lo, int hi, float x, TO& v,
+                                           TO& dt1, TO& dt2) {
+  const float fx = floorf(x);
+  const int ix = (int)fx - 1;
+  float wx[4], dwx[4];
+  bs_w(x - fx, wx);
+  if (DERIV) bs_dw(x - fx, dwx);
+  v = vzero<TO>();
+  dt1 = vzero<TO>();
+  dt2 = vzero<TO>();
+#pragma unroll
+  for (int jx = 0; jx < 4; ++jx) {
+    const int xx = ix + jx;
+    if (xx < lo || xx >= hi) continue;  // within the row's span this is xx outside [0, npsi)
+    const TO cv = cy[xx - lo];
+    v = vfma(wx[jx], cv, v);
+    if (DERIV) {
+      dt2 = vfma(-dwx[jx], cv, dt2);
+      dt1 = vfma(-wx[jx], cdy[xx - lo], dt1);
+    }
+  }
+}
+
 // Sum of (s1, s2) over the T threads of one row group; every thread calls it.
 // The sum is returned to the row's thread t == 0 (and, for T <= 64, to all).
 template <int T>
@@ -115,16 +178,16 @@ __device__ __forceinline__ float2 row_sum2(float2 v, float2* red) {
 
 // ---- K1: rows, forward ------------------------------------------------------
 // Workgroup = ROWS row units (j, ty); loops over the angles of the launch.
-template <int N, int E, typename TO>
+template <int N, int E, typename TO, bool STAGED>
 __global__ __launch_bounds__(kRowThreads) void bh_fwd_rows_k(BhArgs a) {
   constexpr int T = N / E, ROWS = kRowThreads / T;
-  __shared__ float2 lds[ROWS * N];
+  __shared__ float2 lds[ROWS * fft_buf<N>()];
   const int r = threadIdx.x / T, t0 = threadIdx.x % T;
   const int64_t nunits = (int64_t)a.ndist * a.nz;
   const int64_t unit = xcd_tile(blockIdx.x, gridDim.x) * ROWS + r;  // XCD-contiguous row bands
   const bool valid = unit < nunits;
   const int j = valid ? (int)(unit / a.nz) : 0, ty = valid ? (int)(unit % a.nz) : 0;
-  float2* s = lds + r * N;
+  float2* s = lds + r * fft_buf<N>();
   FftTw<N, E> twr;
   fft_load_twiddles<N, E>(twr, t0, a.twx);
   const float2* tab = a.tabx + (int64_t)(j * 2 + 0) * 3 * (N / 2);
@@ -144,12 +207,27 @@ __global__ __launch_bounds__(kRowThreads) void bh_fwd_rows_k(BhArgs a) {
     const TO* cb = (const TO*)a.c + b * plane;
     const int p = a.pidx ? a.pidx[b] : 0;
     const float2* prow = a.prb + (((int64_t)p * a.ndist + j) * a.nz + ty) * N;
+    if constexpr (STAGED) {
+      TO* cy = (TO*)dyn_lds() + r * a.npsi;
+      int lo, hi;
+      row_span<N>(a.npsi, mb, rx, lo, hi);
+      stage_row<TO, T, false>(cb, a.npsi, a.nzpsi, iy, wy, wy, lo, hi, t, cy, cy);
+      __syncthreads();
+#pragma unroll 2
+      for (int e = 0; e < E; ++e) {
+        const int i = t + e * T;
+        TO v, d1, d2;
+        row_gather<TO, false>(cy, cy, lo, hi, bs_coord(i, N, a.npsi, mb, rx), v, d1, d2);
+        s[fpad(dct_slot<N>(i))] = cmul(prow[i], expi_fast(to_c(v)));
+      }
+    } else {
 #pragma unroll 1
-    for (int e = 0; e < E; ++e) {
-      const int i = t + e * T;
-      TO v, d1, d2;
-      s_gather<TO, false>(cb, a.npsi, a.nzpsi, bs_coord(i, N, a.npsi, mb, rx), iy, wy, wy, v, d1, d2);
-      s[dct_slot<N>(i)] = cmul(prow[i], expi_fast(to_c(v)));
+      for (int e = 0; e < E; ++e) {
+        const int i = t + e * T;
+        TO v, d1, d2;
+        s_gather<TO, false>(cb, a.npsi, a.nzpsi, bs_coord(i, N, a.npsi, mb, rx), iy, wy, wy, v, d1, d2);
+        s[fpad(dct_slot<N>(i))] = cmul(prow[i], expi_fast(to_c(v)));
+      }
     }
     __syncthreads();
     fresnel_1d<N, E>(s, t, twr, tab);
@@ -158,7 +236,7 @@ __global__ __launch_bounds__(kRowThreads) void bh_fwd_rows_k(BhArgs a) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int i = t + e * T;
-        dst[i] = s[dct_slot<N>(i)];
+        dst[i] = s[fpad(dct_slot<N>(i))];
       }
     }
     __syncthreads();
@@ -168,7 +246,7 @@ __global__ __launch_bounds__(kRowThreads) void bh_fwd_rows_k(BhArgs a) {
 // ---- K2: columns: D_y, residual, DT_y ---------------------------------------
 template <int N, int E, int C>
 __global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_field) {
-  constexpr int T = N / E, NT = C * T, LS = N + 1;
+  constexpr int T = N / E, NT = C * T, LS = col_stride<N, C>();
   __shared__ float2 lds[C * LS];
   __shared__ double red[NT / 64 > 0 ? NT / 64 : 1];
   const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -187,8 +265,8 @@ __global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_f
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int idx = threadIdx.x + e * NT;
-    const int c = idx % C, y = idx / C;
-    lds[c * LS + dct_slot<N>(y)] = base[y * a.n + c];
+    const int c = idx % C, q = idx / C;  // FFT slot q holds row dct_nat(q)
+    lds[c * LS + fpad(q)] = base[dct_nat<N>(q) * a.n + c];
   }
   __syncthreads();
   float2* s = lds + cc * LS;
@@ -199,7 +277,7 @@ __global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_f
   asm volatile("" : "+v"(tr));
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const int p = tr + e * T;
+    const int p = fpad(tr + e * T);
     const float2 v = s[p];
     const float am = sqrtf(v.x * v.x + v.y * v.y);
     const float q = am - dv[e];
@@ -214,8 +292,8 @@ __global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_f
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int idx = tid + e * NT;
-    const int c = idx % C, y = idx / C;
-    base[y * a.n + c] = lds[c * LS + dct_slot<N>(y)];
+    const int c = idx % C, q = idx / C;
+    base[dct_nat<N>(q) * a.n + c] = lds[c * LS + fpad(q)];
   }
   if constexpr (NT >= 64) {
 #pragma unroll
@@ -241,10 +319,10 @@ __global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_f
 }
 
 // ---- K3: rows, adjoint + epilogue ------------------------------------------
-template <int N, int E, typename TO>
+template <int N, int E, typename TO, bool STAGED>
 __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
   constexpr int T = N / E, ROWS = kRowThreads / T;
-  __shared__ float2 lds[ROWS * N];
+  __shared__ float2 lds[ROWS * fft_buf<N>()];
   __shared__ float2 lacc[ROWS * N];  // probe-gradient accumulator of the row
   __shared__ float2 red[kRowThreads / 64];
   const int r = threadIdx.x / T, t0 = threadIdx.x % T;
@@ -252,7 +330,7 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
   const int64_t unit = xcd_tile(blockIdx.x, gridDim.x) * ROWS + r;  // XCD-contiguous row bands
   const bool valid = unit < nunits;
   const int j = valid ? (int)(unit / a.nz) : 0, ty = valid ? (int)(unit % a.nz) : 0;
-  float2* s = lds + r * N;
+  float2* s = lds + r * fft_buf<N>();
   FftTw<N, E> twr;
   fft_load_twiddles<N, E>(twr, t0, a.twx);
   const float2* tab = a.tabx + (int64_t)(j * 2 + 1) * 3 * (N / 2);
@@ -281,14 +359,7 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = t + e * T;
-      s[dct_slot<N>(i)] = valid ? row[i] : make_float2(0.f, 0.f);
-    }
-    __syncthreads();
-    fresnel_1d<N, E>(s, t, twr, tab);
-    const int p = a.pidx ? a.pidx[b] : 0;
-    if (p != pcur) {
-      flush(pcur);
-      pcur = p;
+      s[fpad(dct_slot<N>(i))] = valid ? row[i] : make_float2(0.f, 0.f);
     }
     const float mb = a.mag[bj], ry = a.pos[2 * bj], rx = a.pos[2 * bj + 1];
     const float y = bs_coord(ty, a.nz, a.nzpsi, mb, ry);
@@ -298,14 +369,31 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
     bs_w(y - fy, wy);
     bs_dw(y - fy, dwy);
     const TO* cb = (const TO*)a.c + b * plane;
+    TO* cy = (TO*)dyn_lds() + 2 * r * a.npsi;
+    TO* cdy = cy + a.npsi;
+    int lo = 0, hi = 0;
+    if constexpr (STAGED) {
+      row_span<N>(a.npsi, mb, rx, lo, hi);
+      stage_row<TO, T, true>(cb, a.npsi, a.nzpsi, iy, wy, dwy, lo, hi, t, cy, cdy);
+    }
+    __syncthreads();
+    fresnel_1d<N, E>(s, t, twr, tab);
+    const int p = a.pidx ? a.pidx[b] : 0;
+    if (p != pcur) {
+      flush(pcur);
+      pcur = p;
+    }
     const float2* prow = a.prb + (((int64_t)p * a.ndist + j) * a.nz + ty) * N;
     float s1 = 0.f, s2 = 0.f;
 #pragma unroll 1
     for (int e = 0; e < E; ++e) {
       const int i = t + e * T;
-      const float2 y12 = s[dct_slot<N>(i)];
+      const float2 y12 = s[fpad(dct_slot<N>(i))];
       TO v, d1, d2;
-      s_gather<TO, true>(cb, a.npsi, a.nzpsi, bs_coord(i, N, a.npsi, mb, rx), iy, wy, dwy, v, d1, d2);
+      if constexpr (STAGED)
+        row_gather<TO, true>(cy, cdy, lo, hi, bs_coord(i, N, a.npsi, mb, rx), v, d1, d2);
+      else
+        s_gather<TO, true>(cb, a.npsi, a.nzpsi, bs_coord(i, N, a.npsi, mb, rx), iy, wy, dwy, v, d1, d2);
       const float2 ex = expi_fast(to_c(v));
       acc[i] = cadd(acc[i], cmulc(y12, ex));                // y12 conj(x12)
       const float2 q = cmulc(cmulc(y12, prow[i]), ex);      // y12 conj(prb) conj(x12)
@@ -334,6 +422,7 @@ __global__ __launch_bounds__(256) void bh_sadj_k(BhArgs a, TO* __restrict__ dpsi
   const int ix = txt * TX + (int)(threadIdx.x % TX);
   const int iy = tyt * TY + (int)(threadIdx.x / TX);
   if (ix >= a.npsi || iy >= a.nzpsi) return;
+  constexpr int kMaxCand = 16;
   TO acc = vzero<TO>();
   for (int j = 0; j < a.ndist; ++j) {
     const int64_t bj = b * a.ndist + j;
@@ -342,15 +431,33 @@ __global__ __launch_bounds__(256) void bh_sadj_k(BhArgs a, TO* __restrict__ dpsi
     bs_candidates(ix, a.n, a.npsi, mb, rx, xlo, xhi);
     bs_candidates(iy, a.nz, a.nzpsi, mb, ry, ylo, yhi);
     const float2* g = a.T + bj * (int64_t)a.nz * a.n;
-    for (int ty = ylo; ty <= yhi; ++ty) {
-      const float wyv = bs_reach(bs_coord(ty, a.nz, a.nzpsi, mb, ry), iy);
-      if (wyv == 0.0f) continue;
-      TO rowv = vzero<TO>();
-      for (int tx = xlo; tx <= xhi; ++tx) {
-        const float wxv = bs_reach(bs_coord(tx, a.n, a.npsi, mb, rx), ix);
-        if (wxv != 0.0f) rowv = vfma(wxv, from_c<TO>(g[(int64_t)ty * a.n + tx]), rowv);
+    if (xhi - xlo < kMaxCand) {
+      // x weights once per distance (they do not depend on the detector row)
+      float wxs[kMaxCand];
+#pragma unroll
+      for (int k = 0; k < kMaxCand; ++k)
+        wxs[k] = xlo + k <= xhi ? bs_reach(bs_coord(xlo + k, a.n, a.npsi, mb, rx), ix) : 0.0f;
+      for (int ty = ylo; ty <= yhi; ++ty) {
+        const float wyv = bs_reach(bs_coord(ty, a.nz, a.nzpsi, mb, ry), iy);
+        if (wyv == 0.0f) continue;
+        const float2* gr = g + (int64_t)ty * a.n + xlo;
+        TO rowv = vzero<TO>();
+#pragma unroll
+        for (int k = 0; k < kMaxCand; ++k)
+          if (wxs[k] != 0.0f) rowv = vfma(wxs[k], from_c<TO>(gr[k]), rowv);
+        acc = vfma(wyv, rowv, acc);
       }
-      acc = vfma(wyv, rowv, acc);
+    } else {
+      for (int ty = ylo; ty <= yhi; ++ty) {
+        const float wyv = bs_reach(bs_coord(ty, a.nz, a.nzpsi, mb, ry), iy);
+        if (wyv == 0.0f) continue;
+        TO rowv = vzero<TO>();
+        for (int tx = xlo; tx <= xhi; ++tx) {
+          const float wxv = bs_reach(bs_coord(tx, a.n, a.npsi, mb, rx), ix);
+          if (wxv != 0.0f) rowv = vfma(wxv, from_c<TO>(g[(int64_t)ty * a.n + tx]), rowv);
+        }
+        acc = vfma(wyv, rowv, acc);
+      }
     }
   }
   const float rho = a.scales[b].y;
@@ -457,16 +564,33 @@ static BhLayout bh_layout(const htamd_prop_plan* pp, const htamd_shift_plan* sp,
   return L;
 }
 
+// LDS of one workgroup: the statically sized buffers plus the staged object
+// rows (one row of npsi values per detector row, two in the adjoint pass);
+// rows are staged only when the total stays within 64 KiB.
+constexpr int64_t kRowLdsBudget = 65536;
+
 template <int N, typename TO>
 static hipError_t launch_rows(bool adjoint, const BhArgs& a, hipStream_t st) {
   constexpr int E = 8;
   constexpr int ROWS = kRowThreads / (N / E);
   const int64_t units = (int64_t)a.ndist * a.nz;
   const int64_t grid = (units + ROWS - 1) / ROWS;
-  if (adjoint)
-    hipLaunchKernelGGL((bh_adj_rows_k<N, E, TO>), dim3((unsigned)grid), dim3(kRowThreads), 0, st, a);
-  else
-    hipLaunchKernelGGL((bh_fwd_rows_k<N, E, TO>), dim3((unsigned)grid), dim3(kRowThreads), 0, st, a);
+  const int64_t stat = (int64_t)ROWS * (fft_buf<N>() + (adjoint ? N : 0)) * 8 + 64;
+  const int64_t dyn = (int64_t)ROWS * a.npsi * (int64_t)sizeof(TO) * (adjoint ? 2 : 1);
+  const bool staged = stat + dyn <= kRowLdsBudget;
+  if (adjoint) {
+    if (staged)
+      hipLaunchKernelGGL((bh_adj_rows_k<N, E, TO, true>), dim3((unsigned)grid), dim3(kRowThreads), (size_t)dyn,
+                         st, a);
+    else
+      hipLaunchKernelGGL((bh_adj_rows_k<N, E, TO, false>), dim3((unsigned)grid), dim3(kRowThreads), 0, st, a);
+  } else {
+    if (staged)
+      hipLaunchKernelGGL((bh_fwd_rows_k<N, E, TO, true>), dim3((unsigned)grid), dim3(kRowThreads), (size_t)dyn,
+                         st, a);
+    else
+      hipLaunchKernelGGL((bh_fwd_rows_k<N, E, TO, false>), dim3((unsigned)grid), dim3(kRowThreads), 0, st, a);
+  }
   return hipGetLastError();
 }
 

@@ -4,6 +4,12 @@
 // complex values per stage and performs E/R radix-R butterflies.  The data
 // lives in LDS between stages (in-place, two workgroup barriers per stage).
 //
+// LDS layout: element i of a transform buffer lives in slot fpad(i) = i +
+// i/16 (one unused slot after every 16 elements), so the strided writes of the
+// early Stockham stages spread over all LDS banks instead of hitting two;
+// fft_buf<N>() is the buffer size in float2 slots.  Every access to a buffer
+// handed to fft_lds goes through fpad.
+//
 // Twiddles: each thread needs one base twiddle w = exp(-2 pi i k/(NS R)) per
 // butterfly and stage (NS > 1); they are read once per kernel from a table
 // tw[m] = exp(-2 pi i m/N) generated in FP64 on the host, held in registers
@@ -14,6 +20,23 @@
 #include "htamd_common.hpp"
 
 namespace htamd {
+
+__device__ __forceinline__ int fpad(int i) { return i + (i >> 4); }
+template <int N>
+constexpr int fft_buf() {
+  return N + N / 16;
+}
+
+// Stride (float2 slots) between the transform buffers of a C-column LDS tile
+// that is filled with consecutive lanes running down the columns, 16/C
+// elements per column per 16 lanes: c * stride = c * 16/C (mod 16) puts the C
+// columns of one 16-lane access on disjoint bank ranges.
+template <int N, int C>
+constexpr int col_stride() {
+  constexpr int B = fft_buf<N>();
+  constexpr int want = C >= 16 ? 1 : 16 / C;
+  return B + ((want - B % 16) % 16 + 16) % 16;
+}
 
 // In-register DFTs.  Forward: X[k] = sum_r a[r] exp(-2 pi i r k / R).
 template <bool INV>
@@ -140,8 +163,14 @@ __device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* w
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const int j = t + b * T;
+    if constexpr ((N / R) % 16 == 0) {
+      const int pj = fpad(j);  // fpad(j + 16 m) = fpad(j) + 17 m
 #pragma unroll
-    for (int r = 0; r < R; ++r) a[b][r] = s[j + r * (N / R)];
+      for (int r = 0; r < R; ++r) a[b][r] = s[pj + r * (N / R / 16) * 17];
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[b][r] = s[fpad(j + r * (N / R))];
+    }
   }
   __syncthreads();
 #pragma unroll
@@ -156,8 +185,14 @@ __device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* w
     }
     dft<R, INV>(a[b]);
     const int base = (j - k) * R + k;
+    if constexpr (NS % 16 == 0) {
+      const int pb = fpad(base);
 #pragma unroll
-    for (int r = 0; r < R; ++r) s[base + r * NS] = a[b][r];
+      for (int r = 0; r < R; ++r) s[pb + r * (NS / 16) * 17] = a[b][r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) s[fpad(base + r * NS)] = a[b][r];
+    }
   }
   __syncthreads();
 }
@@ -171,7 +206,8 @@ __device__ __forceinline__ void stockham(float2* s, int t, const float2* w) {
   }
 }
 
-// Transform s[0..N) in place; every thread of the workgroup must call this
+// Transform the buffer s (fft_buf<N>() slots, element i at s[fpad(i)]) in
+// place; every thread of the workgroup must call this
 // (it contains barriers).  t in [0, N/E) is the thread's index within this FFT
 // and tw its registers from fft_load_twiddles with the same t.
 template <int N, int E, bool INV>

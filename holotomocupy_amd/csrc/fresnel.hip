@@ -29,19 +29,19 @@ __global__ __launch_bounds__(kRowThreads) void fresnel_rows_k(const float2* in, 
                                                               const float2* __restrict__ tab) {
   constexpr int T = N / E;
   constexpr int ROWS = kRowThreads / T;
-  __shared__ float2 lds[ROWS * N];
+  __shared__ float2 lds[ROWS * fft_buf<N>()];
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   const int64_t row = (int64_t)blockIdx.x * ROWS + r;
   const bool valid = row < nrows;
   const int64_t fb = valid ? row / nz : 0, fy = valid ? row % nz : 0;
-  float2* s = lds + r * N;
+  float2* s = lds + r * fft_buf<N>();
   const float2* src = in + fb * in_stride + fy * N;
   FftTw<N, E> twr;
   fft_load_twiddles<N, E>(twr, t, tw);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = t + e * T;
-    s[dct_slot<N>(i)] = valid ? src[i] : make_float2(0.f, 0.f);
+    s[fpad(dct_slot<N>(i))] = valid ? src[i] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   fresnel_1d<N, E>(s, t, twr, tab);
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kRowThreads) void fresnel_rows_k(const float2* in, 
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = t + e * T;
-      dst[i] = s[dct_slot<N>(i)];
+      dst[i] = s[fpad(dct_slot<N>(i))];
     }
   }
 }
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(C*(N / E)) void fresnel_cols_k(float2* data, int64_
                                                             const float2* __restrict__ tab) {
   constexpr int T = N / E;
   constexpr int NT = C * T;
-  constexpr int LS = N + 1;  // padded LDS column stride (bank spread)
+  constexpr int LS = col_stride<N, C>();
   __shared__ float2 lds[C * LS];
   const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const int64_t b = tile / tiles_per_field;
@@ -77,16 +77,16 @@ __global__ __launch_bounds__(C*(N / E)) void fresnel_cols_k(float2* data, int64_
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int idx = threadIdx.x + e * NT;
-    const int c = idx % C, y = idx / C;
-    lds[c * LS + dct_slot<N>(y)] = base[(int64_t)y * n + c];
+    const int c = idx % C, q = idx / C;  // FFT slot q holds row dct_nat(q)
+    lds[c * LS + fpad(q)] = base[(int64_t)dct_nat<N>(q) * n + c];
   }
   __syncthreads();
   fresnel_1d<N, E>(lds + cc * LS, t, twr, tab);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int idx = threadIdx.x + e * NT;
-    const int c = idx % C, y = idx / C;
-    base[(int64_t)y * n + c] = lds[c * LS + dct_slot<N>(y)];
+    const int c = idx % C, q = idx / C;
+    base[(int64_t)dct_nat<N>(q) * n + c] = lds[c * LS + fpad(q)];
   }
 }
 

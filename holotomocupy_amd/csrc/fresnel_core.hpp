@@ -23,12 +23,13 @@ __device__ __forceinline__ void dct_pair_map(float2* s, int t, const float2* __r
     const int k = t + q * T;
     if (k == 0) {
       s[0] = cmul(tab[0], s[0]);
-      s[H] = cmul(tab[H], s[H]);
+      s[fpad(H)] = cmul(tab[H], s[fpad(H)]);
     } else {
-      const float2 v1 = s[k], v2 = s[N - k];
+      const int pk = fpad(k), pm = fpad(N - k);
+      const float2 v1 = s[pk], v2 = s[pm];
       const float2 a = tab[k], b = tab[H + k], c = tab[2 * H + k];
-      s[k] = cadd(cmul(a, v1), cmul(b, v2));
-      s[N - k] = cadd(cmul(c, v1), cmul(a, v2));
+      s[pk] = cadd(cmul(a, v1), cmul(b, v2));
+      s[pm] = cadd(cmul(c, v1), cmul(a, v2));
     }
   }
   __syncthreads();

@@ -47,14 +47,14 @@ template <> __device__ __forceinline__ void st_c<float2>(float2* p, int64_t i, f
 template <int N2, int E, typename TI>
 __global__ __launch_bounds__(256) void tomo_rows_fwd_k(TomoArgs a, const TI* __restrict__ obj) {
   constexpr int T = N2 / E, ROWS = 256 / T;
-  __shared__ float2 lds[ROWS * N2];
+  __shared__ float2 lds[ROWS * fft_buf<N2>()];
   const int n = N2 / 2, h = N2 / 4;
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   const int64_t row = (int64_t)blockIdx.x * ROWS + r;  // (z, i'), i' < n
   const bool valid = row < (int64_t)a.nz * n;
   const int64_t z = valid ? row / n : 0;
   const int ip = valid ? (int)(row % n) : 0;
-  float2* s = lds + r * N2;
+  float2* s = lds + r * fft_buf<N2>();
   FftTw<N2, E> twr;
   fft_load_twiddles<N2, E>(twr, t, a.tw2);
   const TI* src = obj + (z * n + ip) * (int64_t)n;
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void tomo_rows_fwd_k(TomoArgs a, const TI* __r
       const int jp = p - h;
       v = cscale(ld_c<TI>(src, jp), a.phi[ip * n + jp] * sgnf(ip + h + p));
     }
-    s[p] = v;
+    s[fpad(p)] = v;
   }
   __syncthreads();
   fft_lds<N2, E, false>(s, t, twr);
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(256) void tomo_rows_fwd_k(TomoArgs a, const TI* __r
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = t + e * T;
-      dst[k] = cscale(s[k], sgnf(k));
+      dst[k] = cscale(s[fpad(k)], sgnf(k));
     }
   }
 }
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void tomo_rows_fwd_k(TomoArgs a, const TI* __r
 // ---- forward pass B: columns (n non-zero rows embedded in 2n) ---------------
 template <int N2, int E, int C>
 __global__ __launch_bounds__(C*(N2 / E)) void tomo_cols_fwd_k(TomoArgs a) {
-  constexpr int T = N2 / E, NT = C * T, LS = N2 + 1;
+  constexpr int T = N2 / E, NT = C * T, LS = col_stride<N2, C>();
   __shared__ float2 lds[C * LS];
   const int n = N2 / 2, h = N2 / 4;
   const int tiles = N2 / C;
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(C*(N2 / E)) void tomo_cols_fwd_k(TomoArgs a) {
   for (int e = 0; e < E; ++e) {
     const int idx = threadIdx.x + e * NT;
     const int c = idx % C, i = idx / C;
-    lds[c * LS + i] = (i >= h && i < h + n) ? src[(i - h) * N2 + c] : make_float2(0.f, 0.f);
+    lds[c * LS + fpad(i)] = (i >= h && i < h + n) ? src[(i - h) * N2 + c] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   fft_lds<N2, E, false>(lds + cc * LS, t, twr);
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(C*(N2 / E)) void tomo_cols_fwd_k(TomoArgs a) {
   for (int e = 0; e < E; ++e) {
     const int idx = threadIdx.x + e * NT;
     const int c = idx % C, k = idx / C;
-    dst[k * N2 + c] = cscale(lds[c * LS + k], sgnf(k));
+    dst[k * N2 + c] = cscale(lds[c * LS + fpad(k)], sgnf(k));
   }
 }
 
@@ -127,13 +127,13 @@ __device__ __forceinline__ void tomo_taps(const TomoArgs& a, int th, int t, int&
 template <int N, int E, typename TO>
 __global__ __launch_bounds__(256) void tomo_gather_k(TomoArgs a, TO* __restrict__ out) {
   constexpr int T = N / E, ROWS = 256 / T;
-  __shared__ float2 lds[ROWS * N];
+  __shared__ float2 lds[ROWS * fft_buf<N>()];
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   const int64_t line = (int64_t)blockIdx.x * ROWS + r;  // (theta, z)
   const bool valid = line < (int64_t)a.ntheta * a.nz;
   const int th = valid ? (int)(line / a.nz) : 0;
   const int z = valid ? (int)(line % a.nz) : 0;
-  float2* s = lds + r * N;
+  float2* s = lds + r * fft_buf<N>();
   FftTw<N, E> twr;
   fft_load_twiddles<N, E>(twr, t, a.tw1);
   const int n2 = 2 * N;
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(256) void tomo_gather_k(TomoArgs a, TO* __restrict_
       acc = vfma(wy, racc, acc);
     }
     // c1d . (gather / n), c1d[k] = (-1)^(k+1)
-    s[tt] = cscale(acc, -sgnf(tt) / (float)N);
+    s[fpad(tt)] = cscale(acc, -sgnf(tt) / (float)N);
   }
   __syncthreads();
   fft_lds<N, E, true>(s, t, twr);
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(256) void tomo_gather_k(TomoArgs a, TO* __restrict_
     TO* dst = out + ((int64_t)th * a.nz + z) * N;
     for (int e = 0; e < E; ++e) {
       const int tt = t + e * T;
-      st_c<TO>(dst, tt, cscale(s[tt], -sgnf(tt) / (4.0f * N)));
+      st_c<TO>(dst, tt, cscale(s[fpad(tt)], -sgnf(tt) / (4.0f * N)));
     }
   }
 }
@@ -180,19 +180,19 @@ __global__ __launch_bounds__(256) void tomo_gather_k(TomoArgs a, TO* __restrict_
 template <int N, int E, typename TI>
 __global__ __launch_bounds__(256) void tomo_scatter_k(TomoArgs a, const TI* __restrict__ data) {
   constexpr int T = N / E, ROWS = 256 / T;
-  __shared__ float2 lds[ROWS * N];
+  __shared__ float2 lds[ROWS * fft_buf<N>()];
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   const int64_t line = (int64_t)blockIdx.x * ROWS + r;
   const bool valid = line < (int64_t)a.ntheta * a.nz;
   const int th = valid ? (int)(line / a.nz) : 0;
   const int z = valid ? (int)(line % a.nz) : 0;
-  float2* s = lds + r * N;
+  float2* s = lds + r * fft_buf<N>();
   FftTw<N, E> twr;
   fft_load_twiddles<N, E>(twr, t, a.tw1);
   const TI* src = data + ((int64_t)th * a.nz + z) * N;
   for (int e = 0; e < E; ++e) {
     const int tt = t + e * T;
-    s[tt] = valid ? cscale(ld_c<TI>(src, tt), -sgnf(tt)) : make_float2(0.f, 0.f);
+    s[fpad(tt)] = valid ? cscale(ld_c<TI>(src, tt), -sgnf(tt)) : make_float2(0.f, 0.f);
   }
   __syncthreads();
   fft_lds<N, E, false>(s, t, twr);
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(256) void tomo_scatter_k(TomoArgs a, const TI* __re
   const int L = 2 * a.m + 1;
   for (int e = 0; e < E; ++e) {
     const int tt = t + e * T;
-    const float2 v = cscale(s[tt], -sgnf(tt));
+    const float2 v = cscale(s[fpad(tt)], -sgnf(tt));
     int by, bx;
     double x0, y0;
     tomo_taps(a, th, tt, by, bx, x0, y0);
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(256) void tomo_scatter_k(TomoArgs a, const TI* __re
 // ---- adjoint pass B': columns of the grid, keep the n central rows ----------
 template <int N2, int E, int C>
 __global__ __launch_bounds__(C*(N2 / E)) void tomo_cols_adj_k(TomoArgs a) {
-  constexpr int T = N2 / E, NT = C * T, LS = N2 + 1;
+  constexpr int T = N2 / E, NT = C * T, LS = col_stride<N2, C>();
   __shared__ float2 lds[C * LS];
   const int n = N2 / 2, h = N2 / 4;
   const int tiles = N2 / C;
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(C*(N2 / E)) void tomo_cols_adj_k(TomoArgs a) {
   for (int e = 0; e < E; ++e) {
     const int idx = threadIdx.x + e * NT;
     const int c = idx % C, k = idx / C;
-    lds[c * LS + k] = cscale(src[k * N2 + c], sgnf(k + k0 + c));
+    lds[c * LS + fpad(k)] = cscale(src[k * N2 + c], sgnf(k + k0 + c));
   }
   __syncthreads();
   fft_lds<N2, E, true>(lds + cc * LS, t, twr);
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(C*(N2 / E)) void tomo_cols_adj_k(TomoArgs a) {
   for (int e = 0; e < E; ++e) {
     const int idx = threadIdx.x + e * NT;
     const int c = idx % C, i = idx / C;
-    if (i >= h && i < h + n) dst[(i - h) * N2 + c] = cscale(lds[c * LS + i], sc * sgnf(i));
+    if (i >= h && i < h + n) dst[(i - h) * N2 + c] = cscale(lds[c * LS + fpad(i)], sc * sgnf(i));
   }
 }
 
@@ -263,21 +263,21 @@ __global__ __launch_bounds__(C*(N2 / E)) void tomo_cols_adj_k(TomoArgs a) {
 template <int N2, int E, typename TO>
 __global__ __launch_bounds__(256) void tomo_rows_adj_k(TomoArgs a, TO* __restrict__ out) {
   constexpr int T = N2 / E, ROWS = 256 / T;
-  __shared__ float2 lds[ROWS * N2];
+  __shared__ float2 lds[ROWS * fft_buf<N2>()];
   const int n = N2 / 2, h = N2 / 4;
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   const int64_t row = (int64_t)blockIdx.x * ROWS + r;
   const bool valid = row < (int64_t)a.nz * n;
   const int64_t z = valid ? row / n : 0;
   const int ip = valid ? (int)(row % n) : 0;
-  float2* s = lds + r * N2;
+  float2* s = lds + r * fft_buf<N2>();
   FftTw<N2, E> twr;
   fft_load_twiddles<N2, E>(twr, t, a.tw2);
   const float2* src = a.W + (z * n + ip) * (int64_t)N2;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int k = t + e * T;
-    s[k] = src[k];
+    s[fpad(k)] = src[k];
   }
   __syncthreads();
   fft_lds<N2, E, true>(s, t, twr);
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(256) void tomo_rows_adj_k(TomoArgs a, TO* __restric
       const int p = t + e * T;
       if (p >= h && p < h + n) {
         const int jp = p - h;
-        st_c<TO>(dst, jp, cscale(s[p], sc * sgnf(p) * a.phi[ip * n + jp]));
+        st_c<TO>(dst, jp, cscale(s[fpad(p)], sc * sgnf(p) * a.phi[ip * n + jp]));
       }
     }
   }
@@ -302,8 +302,8 @@ __global__ void tomo_zero_k(float2* __restrict__ p, int64_t count) {
 }
 
 // ---- host dispatch -----------------------------------------------------------
-// columns per workgroup of the column passes: at most 16, at most 64 KiB of
-// LDS (8192 / N2 columns) and at most 1024 threads
+// columns per workgroup of the column passes: at most 16, at most about 70 KiB
+// of LDS (8192 / N2 padded columns) and at most 1024 threads
 template <int N2>
 constexpr int tomo_cols_c() {
   constexpr int T = N2 / 8;
