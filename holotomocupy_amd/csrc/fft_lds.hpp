@@ -75,23 +75,82 @@ __device__ __forceinline__ void dft8(float2* a) {
   a[3] = cadd(e3, o3); a[7] = csub(e3, o3);
 }
 
+// multiply by W16^m = exp(-+ 2 pi i m / 16), m a compile-time constant
+template <int m, bool INV>
+__device__ __forceinline__ float2 mul_w16(float2 v) {
+  constexpr int q = m & 15;
+  if constexpr (q == 0) {
+    return v;
+  } else if constexpr (q == 4) {
+    return mul_mi<INV>(v);
+  } else if constexpr (q == 8) {
+    return make_float2(-v.x, -v.y);
+  } else {
+    // cos(2 pi q / 16), q = 0..15
+    constexpr double kC[16] = {1.0, 0.92387953251128674, 0.70710678118654752, 0.38268343236508977,
+                               0.0, -0.38268343236508977, -0.70710678118654752, -0.92387953251128674,
+                               -1.0, -0.92387953251128674, -0.70710678118654752, -0.38268343236508977,
+                               0.0, 0.38268343236508977, 0.70710678118654752, 0.92387953251128674};
+    constexpr float c = (float)kC[q];
+    constexpr float sn = (float)kC[(q + 12) & 15];  // sin(x) = cos(x - pi/2)
+    constexpr float s = INV ? sn : -sn;
+    return make_float2(c * v.x - s * v.y, c * v.y + s * v.x);
+  }
+}
+
+// radix 16 as 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2
+template <bool INV>
+__device__ __forceinline__ void dft16(float2* a) {
+  float2 y[16];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    float2 b0 = a[n2], b1 = a[4 + n2], b2 = a[8 + n2], b3 = a[12 + n2];
+    dft4<INV>(b0, b1, b2, b3);  // over n1 -> k1
+    y[n2 * 4 + 0] = b0;
+    y[n2 * 4 + 1] = b1;
+    y[n2 * 4 + 2] = b2;
+    y[n2 * 4 + 3] = b3;
+  }
+  // W16^(n2 k1)
+  y[5] = mul_w16<1, INV>(y[5]);
+  y[6] = mul_w16<2, INV>(y[6]);
+  y[7] = mul_w16<3, INV>(y[7]);
+  y[9] = mul_w16<2, INV>(y[9]);
+  y[10] = mul_w16<4, INV>(y[10]);
+  y[11] = mul_w16<6, INV>(y[11]);
+  y[13] = mul_w16<3, INV>(y[13]);
+  y[14] = mul_w16<6, INV>(y[14]);
+  y[15] = mul_w16<9, INV>(y[15]);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    float2 b0 = y[k1], b1 = y[4 + k1], b2 = y[8 + k1], b3 = y[12 + k1];
+    dft4<INV>(b0, b1, b2, b3);  // over n2 -> k2
+    a[k1] = b0;
+    a[k1 + 4] = b1;
+    a[k1 + 8] = b2;
+    a[k1 + 12] = b3;
+  }
+}
+
 template <int R, bool INV>
 __device__ __forceinline__ void dft(float2* a) {
-  if constexpr (R == 2) {
+  if constexpr (R == 16) {
+    dft16<INV>(a);
+  } else if constexpr (R == 2) {
     dft2<INV>(a[0], a[1]);
   } else if constexpr (R == 4) {
     dft4<INV>(a[0], a[1], a[2], a[3]);
   } else if constexpr (R == 8) {
     dft8<INV>(a);
   } else {
-    static_assert(R == 2 || R == 4 || R == 8, "unsupported radix");
+    static_assert(R == 2 || R == 4 || R == 8 || R == 16, "unsupported radix");
   }
 }
 
 // ---- compile-time stage schedule -------------------------------------------
-// Stage radix: at most 8 (in-register DFTs up to radix 8); a thread owning
-// E > 8 values performs E/R butterflies per stage.
-constexpr int kMaxRadix = 8;
+// Stage radix: at most 16 (in-register DFTs up to radix 16); a thread owning
+// E > 16 values performs E/R butterflies per stage.
+constexpr int kMaxRadix = 16;
 template <int N, int E, int NS>
 constexpr int stage_radix() {
   constexpr int R0 = E < kMaxRadix ? E : kMaxRadix;
@@ -137,7 +196,7 @@ __device__ __forceinline__ void fft_load_twiddles(FftTw<N, E>& r, int t, const f
   load_tw_rec<N, E, 1, 0>(r.w, t, tw);
 }
 
-// powers p[r] = w^r, r = 1..R-1 (p[0] unused), products of depth <= 3
+// powers p[r] = w^r, r = 1..R-1 (p[0] unused), products of depth <= 4
 template <int R, bool INV>
 __device__ __forceinline__ void tw_powers(float2 w, float2* p) {
   if (INV) w.y = -w.y;
@@ -151,6 +210,11 @@ __device__ __forceinline__ void tw_powers(float2 w, float2* p) {
     p[5] = cmul(p[4], w);
     p[6] = cmul(p[3], p[3]);
     p[7] = cmul(p[4], p[3]);
+  }
+  if constexpr (R > 8) {
+    p[8] = cmul(p[4], p[4]);
+#pragma unroll
+    for (int r = 1; r < 8; ++r) p[8 + r] = cmul(p[8], p[r]);
   }
 }
 
@@ -178,8 +242,10 @@ __device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* w
     const int j = t + b * T;
     const int k = j & (NS - 1);
     if constexpr (NS > 1) {
+      // butterflies b of one thread share k (and the twiddles) when T % NS == 0
+      constexpr bool kShared = (T % NS) == 0;
       float2 p[R];
-      tw_powers<R, INV>(w[b], p);
+      tw_powers<R, INV>(w[kShared ? 0 : b], p);
 #pragma unroll
       for (int r = 1; r < R; ++r) a[b][r] = cmul(a[b][r], p[r]);
     }
