@@ -10,3 +10,4 @@ __version__ = "0.1.0"
 from .propagation import Propagation  # noqa: F401
 from .shift import Shift  # noqa: F401
 from .tomo import Tomo  # noqa: F401
+from .chunking import Chunking  # noqa: F401

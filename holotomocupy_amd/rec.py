@@ -323,6 +323,78 @@ class Rec:
             y = self.F[i](y)
         return float(self.F0(y, data))
 
+    # ------------------------------------------------------------------ probe fit
+    def gradient_prbfit(self, grad_prb, prb):
+        """Adds the gradient of lam_prbfit/prb_size || |D prb| - ref ||^2 to
+        grad_prb, scaled by rho_prb^2 (rec_mpi.py:365-378); self.ref is the
+        [ndist, nz, n] float32 reference amplitude."""
+        if self.lam_prbfit == 0:
+            return
+        ref = _lib.as_device(self.ref, torch.float32, self.device)
+        for j in range(self.ndist):
+            tmp = self.cl_prop.D(prb[j:j + 1], j)
+            res = torch.empty_like(tmp)
+            _lib.call("htamd_cas_residual", _lib.ptr(tmp), _lib.ptr(ref[j:j + 1]), _lib.ptr(res), tmp.numel(),
+                      float(2 * self.lam_prbfit / self.prb_size), _lib.stream_ptr())
+            grad_prb[j:j + 1] += self.cl_prop.DT(res, j) * self.rho_sq["prb"]
+
+    def hessian_prbfit(self, prb, dprb1, dprb2):
+        """Bilinear form of the probe-fit term (rec_mpi.py:380-398), as a float."""
+        if self.lam_prbfit == 0:
+            return 0
+        ref = _lib.as_device(self.ref, torch.float32, self.device)
+        out = 0.0
+        for j in range(self.ndist):
+            Dp = self.cl_prop.D(prb[j:j + 1], j)
+            D1 = self.cl_prop.D(dprb1[j:j + 1], j)
+            D2 = self.cl_prop.D(dprb2[j:j + 1], j)
+            out += 2 * float(self._reduce(2, Dp, ref[j:j + 1], D1, D2))
+        return self.lam_prbfit * out / self.prb_size
+
+    # ------------------------------------------------------------------ tomography
+    def _tomo(self):
+        """Tomo(nobj, nchunk, theta, mask_r) of the reconstruction (rec_mpi.py:71-74);
+        needs args.theta (and args.mask_r, default 0)."""
+        if getattr(self, "cl_tomo", None) is None:
+            from .tomo import Tomo
+            self.cl_tomo = Tomo(self.nobj, getattr(self, "nchunk", self.nzobj), self.theta,
+                                getattr(self, "mask_r", 0))
+        return self.cl_tomo
+
+    def _chunking(self):
+        """Chunking pool of rec_mpi.py:39-42 (16 nchunk nobj max(nobj, nzobj, ntheta) x 8 bytes)."""
+        if getattr(self, "cl_chunk", None) is None:
+            from .chunking import Chunking
+            nchunk = getattr(self, "nchunk", self.nzobj)
+            nbytes = 16 * nchunk * self.nobj * max(self.nobj, self.nzobj, self.ntheta) * 8
+            self.cl_chunk = Chunking(nbytes, nchunk)
+        return self.cl_chunk
+
+    def fwd_tomo(self, obj, out):
+        """out [ntheta, nzobj, nobj] = R(obj [nzobj, nobj, nobj]) chunk by chunk
+        over the slices (rec_mpi.py:498-507); host or device arrays."""
+        cl = self._chunking()
+        tomo = self._tomo()
+
+        @cl.gpu_batch(axis_out=1, axis_inp=0, nout=1)
+        def _fwd_tomo(self, out, obj):
+            out[:] = tomo.R(obj)
+
+        _fwd_tomo(self, out, obj)
+        return out
+
+    def gF4(self, gradu, gradproj):
+        """gradu [nzobj, nobj, nobj] = RT(gradproj [ntheta, nzobj, nobj]) chunk by
+        chunk (rec_mpi.py:355-362)."""
+        cl = self._chunking()
+        tomo = self._tomo()
+
+        @cl.gpu_batch(axis_out=0, axis_inp=1, nout=1)
+        def _gF4(self, gradu, gradproj):
+            gradu[:] = tomo.RT(gradproj)
+
+        _gF4(self, gradu, gradproj)
+
     def forward(self, vars, st=0):
         """F1(F2(F3(x))) [B, ndist, nz, n]; its modulus is the synthetic data
         (rec_mpi.py:882-896)."""

@@ -208,6 +208,23 @@ int htamd_bh_gradient(const htamd_bh_plan* plan, int64_t nangles, const void* pr
                       float* gpos, void* gprb, double* objective, void* workspace,
                       int64_t workspace_bytes, void* stream);
 
+/* ---- BLAS-1 helpers of the BH iteration ----------------------------------------
+ * Arrays are float streams of n floats (a complex64 array of m elements is
+ * n = 2m); 16-byte aligned pointers take a vectorised path.  Reductions are deterministic
+ * (fixed FP64 partial sums) into a device double; accumulate != 0 adds to it. */
+/* Replaces linear_batch / mulc_batch on device arrays (src/holotomocupy/chunking.py:251-290):
+ * out = a x + b y (y NULL: out = a x); out may equal x or y. */
+int htamd_axpby(const void* x, const void* y, float a, float b, void* out, int64_t n, void* stream);
+int htamd_blas1_workspace_size(int64_t* bytes);
+/* Replaces redot / redot_batch (src/holotomocupy/utils.py:96-103, chunking.py:234-247):
+ * result (+)= Re<x, y> = sum x_k y_k. */
+int htamd_redot(const void* x, const void* y, int64_t n, double* result, int accumulate, void* workspace,
+                int64_t workspace_bytes, void* stream);
+/* Replaces linear_redot_batch (src/holotomocupy/chunking.py:264-278): x = a x + b y,
+ * then result (+)= Re<y, x>, in one pass. */
+int htamd_axpby_redot(void* x, const void* y, float a, float b, int64_t n, double* result, int accumulate,
+                      void* workspace, int64_t workspace_bytes, void* stream);
+
 /* ---- USFFT Radon transform ---------------------------------------------------
  * Geometry of Tomo(n, nz, theta, mask_r) (src/holotomocupy/tomo.py:12-54): slices
  * n x n, n a power of two in [8, 1024], ntheta angles (radians; rounded to
