@@ -36,6 +36,10 @@ def parse():
     p.add_argument("--path", choices=["auto", "plain", "fused", "served"], default="auto")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison run")
+    p.add_argument("--op", choices=["prop", "tomo", "shift"], default=None,
+                   help="operator benchmark instead of the BH gradient: prop = D then DT, 16 fields of "
+                        "256^2 at 4 distances; tomo = R and RT at 256^3, 384 angles; shift = coeff, S and "
+                        "Sadj at 384 x 256^2")
     p.add_argument("--copy-calibration", action="store_true",
                    help="first run a 256 MiB device-to-device copy 3 times (a known-size kernel for "
                         "scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -247,6 +251,103 @@ def cpu_baseline(a, host):
                       f"oracle cascade (scipy.fft workers={cores}), one run of {dt:.1f} s"}
 
 
+def _timed(fn, steps, warmup):
+    """Mean ms per call of fn over `steps` calls after `warmup`, with events on
+    the current stream."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(steps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / steps
+
+
+def _op_line(metric, value, unit, ms, nbytes, workload, a, parts):
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return {"metric": metric, "value": round(value, 3), "unit": unit, "n_gpus": 1, "steps": a.steps,
+            "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "c64", "data": "synthetic (seeded)",
+            "config": {"workload": workload},
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                         "bytes_per_step": int(nbytes)},
+            "parts_ms": {k: round(v, 4) for k, v in parts.items()}}
+
+
+def op_prop(a):
+    """D then DT on 16 complex64 fields of 256^2 at 4 distances (Fresnel numbers
+    1e-3, 1.2e-3, 1.5e-3, 2e-3 with voxelsize = wavelength = 1).  Bytes per
+    field and operator: 4F (row pass and column pass, F = 8 nz n)."""
+    import numpy as np
+    import torch
+    from holotomocupy_amd import Propagation
+    n, B, nd = 256, 16, 4
+    prop = Propagation(n, n, B, nd, 1.0, 1.0, 1.0 / np.array([1e-3, 1.2e-3, 1.5e-3, 2e-3]))
+    g = torch.Generator(device="cuda").manual_seed(1001)
+    x = torch.randn((B, n, n), dtype=torch.complex64, device="cuda", generator=g)
+    y = torch.empty_like(x)
+
+    def step():
+        for j in range(nd):
+            prop.apply_into(x, y, j, False)
+            prop.apply_into(y, y, j, True)
+
+    ms = _timed(step, a.steps, a.warmup)
+    F = 8 * n * n
+    nbytes = nd * 2 * B * 4 * F
+    return _op_line("D+DT field evals/s (16 x 256^2, 4 distances)", nd * 2 * B / (ms * 1e-3), "evals/s", ms,
+                    nbytes, "Fresnel D then DT, 16 fields 256^2, 4 distances", a, {"step": ms})
+
+
+def op_tomo(a):
+    """R and RT at 256^3 with 384 angles; bytes per slice 8 (13 n^2 + ntheta n)
+    for each operator."""
+    import numpy as np
+    import torch
+    from holotomocupy_amd import Tomo
+    n, nth = 256, 384
+    tomo = Tomo(n, n, np.arange(nth) * np.pi / nth, 0)
+    g = torch.Generator(device="cuda").manual_seed(1002)
+    u = torch.randn((n, n, n), dtype=torch.complex64, device="cuda", generator=g)
+    d = torch.randn((nth, n, n), dtype=torch.complex64, device="cuda", generator=g)
+    ms_r = _timed(lambda: tomo.R(u), a.steps, a.warmup)
+    ms_rt = _timed(lambda: tomo.RT(d), a.steps, a.warmup)
+    nbytes = 2 * n * 8 * (13 * n * n + nth * n)
+    ms = ms_r + ms_rt
+    return _op_line("R+RT evals/s (256^3, 384 angles)", 1.0 / (ms * 1e-3), "evals/s", ms, nbytes,
+                    "USFFT Radon R and RT, 256 slices of 256^2, 384 angles", a, {"R": ms_r, "RT": ms_rt})
+
+
+def op_shift(a):
+    """coeff, S and Sadj on 384 complex64 planes of 256^2 (m = 1, shifts
+    U[-5, 5]); bytes: coeff 2 Fo, S Fo + F, Sadj F + Fo per plane."""
+    import numpy as np
+    import torch
+    from holotomocupy_amd import Shift
+    n, B = 256, 384
+    sh = Shift(n, n, n, n, "complex64")
+    rng = np.random.default_rng(1002)
+    g = torch.Generator(device="cuda").manual_seed(1002)
+    psi = torch.randn((B, n, n), dtype=torch.complex64, device="cuda", generator=g)
+    r = torch.from_numpy(rng.uniform(-5, 5, (B, 2)).astype(np.float32)).cuda()
+    m = torch.ones(B, dtype=torch.float32, device="cuda")
+    c = sh.coeff(psi)
+    ms_c = _timed(lambda: sh.coeff(psi), a.steps, a.warmup)
+    ms_s = _timed(lambda: sh.S(c, r, m), a.steps, a.warmup)
+    ms_a = _timed(lambda: sh.Sadj(psi, r, m), a.steps, a.warmup)
+    F = 8 * n * n
+    nbytes = B * (2 * F + 2 * F + 2 * F)
+    ms = ms_c + ms_s + ms_a
+    return _op_line("coeff+S+Sadj plane evals/s (384 x 256^2)", B / (ms * 1e-3), "evals/s", ms, nbytes,
+                    "B-spline prefilter, shift and adjoint shift, 384 planes of 256^2", a,
+                    {"coeff": ms_c, "S": ms_s, "Sadj": ms_a})
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -255,6 +356,9 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
+    if a.op:
+        print(json.dumps({"prop": op_prop, "tomo": op_tomo, "shift": op_shift}[a.op](a)), flush=True)
+        return
     rec, vars_, data, host = make_inputs(a)
     grads = {"prb": torch.zeros_like(vars_["prb"]), "proj": torch.empty_like(vars_["proj"]),
              "pos": torch.empty_like(vars_["pos"])}

@@ -8,10 +8,12 @@
 // transforms run as two LDS FFT passes (rows, then columns; only the n
 // non-zero rows of the embedded slice are transformed along x), with the sign
 // modulations, zero-embedding, cropping and deapodisation folded into the
-// loads and stores.  The polar gather / scatter works one (angle, slice) line
-// per row group with the 9x9 Gaussian taps evaluated in FP64; the line's 1-D
-// FFT is fused.  The scatter uses float atomics on the grid, as the reference
-// kernel does (its summation order is not fixed).
+// loads and stores.  The 9x9 Gaussian gridding taps of every polar sample are
+// built once per plan in FP64 (stored FP32): the forward gathers them per
+// (angle, slice) line with the line's 1-D FFT fused; the adjoint uses the same
+// taps transposed (one CSR row per grid point) and gathers onto the grid from
+// a slice-contiguous copy of the transformed sinogram -- the reference's atomic
+// scatter in a fixed summation order.
 #include <cmath>
 #include <string>
 #include <vector>
@@ -25,13 +27,19 @@ namespace htamd {
 
 struct TomoArgs {
   int n = 0, nz = 0, ntheta = 0, m = 0;
-  double mu = 0.0;
-  const double* cs = nullptr;      // [ntheta] cos(theta), sin at cs[ntheta + k]
   const float* phi = nullptr;      // [n, n] deapodisation (mask and scale included)
   const float2* tw2 = nullptr;     // twiddles of length 2n
   const float2* tw1 = nullptr;     // twiddles of length n
+  // gridding taps of polar sample s = theta n + t (built in FP64 on the host):
+  const int2* tb = nullptr;        // [ntheta n] first tap (bx, by) on the 2n grid, before wrapping
+  const float* tw = nullptr;       // [2L][ntheta n]: wx[i] at tw[i S + s], wy[i] at tw[(L + i) S + s]
+  // the same taps transposed, one row per grid point g = row 2n + col (CSR):
+  const int* rp = nullptr;         // [4 n^2 + 1]
+  const int* ci = nullptr;         // [nnz] sample index s
+  const float* cw = nullptr;       // [nnz] weight wy wx
   float2* W = nullptr;             // [nz, n, 2n]
   float2* G = nullptr;             // [nz, 2n, 2n]
+  float2* Sh = nullptr;            // [ntheta n, nz] transformed sinogram (RT)
 };
 
 __device__ __forceinline__ float sgnf(int k) { return (k & 1) ? -1.0f : 1.0f; }
@@ -111,18 +119,6 @@ __global__ __launch_bounds__(C*(N2 / E)) void tomo_cols_fwd_k(TomoArgs a) {
   }
 }
 
-// 9x9 (2m+1)^2 Gaussian taps of polar sample t of angle `th`: rows / cols on
-// the 2n grid and separable FP64 weights (cuda_kernels.py:17-51).
-__device__ __forceinline__ void tomo_taps(const TomoArgs& a, int th, int t, int& by, int& bx,
-                                          double& x0, double& y0) {
-  const int n = a.n;
-  const double c = a.cs[th], sn = a.cs[a.ntheta + th];
-  x0 = (double)(t - n / 2) / n * c;
-  y0 = -(double)(t - n / 2) / n * sn;
-  bx = (int)floor(2.0 * n * x0) - a.m;
-  by = (int)floor(2.0 * n * y0) - a.m;
-}
-
 // ---- forward pass C: gather + line IFFT ------------------------------------
 template <int N, int E, typename TO>
 __global__ __launch_bounds__(256) void tomo_gather_k(TomoArgs a, TO* __restrict__ out) {
@@ -137,23 +133,19 @@ __global__ __launch_bounds__(256) void tomo_gather_k(TomoArgs a, TO* __restrict_
   FftTw<N, E> twr;
   fft_load_twiddles<N, E>(twr, t, a.tw1);
   const int n2 = 2 * N;
-  const double cy = M_PI / a.mu, cx = -M_PI * M_PI / a.mu, inv2n = 1.0 / n2;
   const float2* g = a.G + (int64_t)z * n2 * n2;
   const int L = 2 * a.m + 1;
+  const int64_t S = (int64_t)a.ntheta * N;
   for (int e = 0; e < E; ++e) {
     const int tt = t + e * T;
-    int by, bx;
-    double x0, y0;
-    tomo_taps(a, th, tt, by, bx, x0, y0);
+    const int64_t si = (int64_t)th * N + tt;
+    const int2 b = a.tb[si];
+    const int bx = b.x, by = b.y;
     float wx[16];
-    for (int i0 = 0; i0 < L; ++i0) {
-      const double w0 = (bx + i0) * inv2n - x0;
-      wx[i0] = (float)exp(cx * w0 * w0);
-    }
+    for (int i0 = 0; i0 < L; ++i0) wx[i0] = a.tw[i0 * S + si];
     float2 acc = make_float2(0.f, 0.f);
     for (int i1 = 0; i1 < L; ++i1) {
-      const double w1 = (by + i1) * inv2n - y0;
-      const float wy = (float)(cy * exp(cx * w1 * w1));
+      const float wy = a.tw[(L + i1) * S + si];
       const int row = (N + by + i1 + n2) % n2;
       float2 racc = make_float2(0.f, 0.f);
       for (int i0 = 0; i0 < L; ++i0) {
@@ -176,13 +168,13 @@ __global__ __launch_bounds__(256) void tomo_gather_k(TomoArgs a, TO* __restrict_
   }
 }
 
-// ---- adjoint pass C': line FFT + scatter -----------------------------------
+// ---- adjoint pass C': line FFT into the z-contiguous sinogram ---------------
 template <int N, int E, typename TI>
-__global__ __launch_bounds__(256) void tomo_scatter_k(TomoArgs a, const TI* __restrict__ data) {
+__global__ __launch_bounds__(256) void tomo_linefft_k(TomoArgs a, const TI* __restrict__ data) {
   constexpr int T = N / E, ROWS = 256 / T;
   __shared__ float2 lds[ROWS * fft_buf<N>()];
   const int r = threadIdx.x / T, t = threadIdx.x % T;
-  const int64_t line = (int64_t)blockIdx.x * ROWS + r;
+  const int64_t line = (int64_t)blockIdx.x * ROWS + r;  // (theta, z), z fastest
   const bool valid = line < (int64_t)a.ntheta * a.nz;
   const int th = valid ? (int)(line / a.nz) : 0;
   const int z = valid ? (int)(line % a.nz) : 0;
@@ -196,35 +188,48 @@ __global__ __launch_bounds__(256) void tomo_scatter_k(TomoArgs a, const TI* __re
   }
   __syncthreads();
   fft_lds<N, E, false>(s, t, twr);
-  if (!valid) return;  // no barriers below
-  const int n2 = 2 * N;
-  const double cy = M_PI / a.mu, cx = -M_PI * M_PI / a.mu, inv2n = 1.0 / n2;
-  float2* g = a.G + (int64_t)z * n2 * n2;
-  const int L = 2 * a.m + 1;
-  for (int e = 0; e < E; ++e) {
-    const int tt = t + e * T;
-    const float2 v = cscale(s[fpad(tt)], -sgnf(tt));
-    int by, bx;
-    double x0, y0;
-    tomo_taps(a, th, tt, by, bx, x0, y0);
-    float wx[16];
-    for (int i0 = 0; i0 < L; ++i0) {
-      const double w0 = (bx + i0) * inv2n - x0;
-      wx[i0] = (float)exp(cx * w0 * w0);
-    }
-    for (int i1 = 0; i1 < L; ++i1) {
-      const double w1 = (by + i1) * inv2n - y0;
-      const float wy = (float)(cy * exp(cx * w1 * w1));
-      const int row = (N + by + i1 + n2) % n2;
-      for (int i0 = 0; i0 < L; ++i0) {
-        const int col = (N + bx + i0 + n2) % n2;
-        const float w = wy * wx[i0];
-        float2* p = g + (int64_t)row * n2 + col;
-        atomicAdd(&p->x, w * v.x);
-        atomicAdd(&p->y, w * v.y);
-      }
+  if (valid) {
+    for (int e = 0; e < E; ++e) {
+      const int tt = t + e * T;
+      a.Sh[((int64_t)th * N + tt) * a.nz + z] = cscale(s[fpad(tt)], -sgnf(tt));
     }
   }
+}
+
+// ---- adjoint pass C'': gather onto the grid (transpose of the forward taps) ---
+// Thread per grid point and kZB consecutive slices; the contributions of a
+// grid point are summed in sample order (deterministic, no atomics).
+constexpr int kZB = 16;
+template <bool FULL>
+__global__ __launch_bounds__(256) void tomo_grid_gather_k(TomoArgs a) {
+  const int64_t n4 = 4LL * a.n * a.n;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n4) return;
+  const int z0 = blockIdx.y * kZB;
+  float2 acc[kZB];
+#pragma unroll
+  for (int q = 0; q < kZB; ++q) acc[q] = make_float2(0.f, 0.f);
+  const int k0 = a.rp[g], k1 = a.rp[g + 1];
+  for (int k = k0; k < k1; ++k) {
+    const float w = a.cw[k];
+    const float2* src = a.Sh + (int64_t)a.ci[k] * a.nz + z0;
+    if (FULL) {  // kZB whole slices, 16-byte aligned (nz % kZB == 0): pairs as float4
+      const float4* s4 = (const float4*)src;
+#pragma unroll
+      for (int q = 0; q < kZB / 2; ++q) {
+        const float4 v = s4[q];
+        acc[2 * q] = vfma(w, make_float2(v.x, v.y), acc[2 * q]);
+        acc[2 * q + 1] = vfma(w, make_float2(v.z, v.w), acc[2 * q + 1]);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < kZB; ++q)
+        if (z0 + q < a.nz) acc[q] = vfma(w, src[q], acc[q]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kZB; ++q)
+    if (FULL || z0 + q < a.nz) a.G[(int64_t)(z0 + q) * n4 + g] = acc[q];
 }
 
 // ---- adjoint pass B': columns of the grid, keep the n central rows ----------
@@ -295,12 +300,6 @@ __global__ __launch_bounds__(256) void tomo_rows_adj_k(TomoArgs a, TO* __restric
   }
 }
 
-__global__ void tomo_zero_k(float2* __restrict__ p, int64_t count) {
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count;
-       k += (int64_t)gridDim.x * blockDim.x)
-    p[k] = make_float2(0.f, 0.f);
-}
-
 // ---- host dispatch -----------------------------------------------------------
 // columns per workgroup of the column passes: at most 16, at most about 70 KiB
 // of LDS (8192 / N2 padded columns) and at most 1024 threads
@@ -335,16 +334,19 @@ template <int N2, typename TIO>
 static hipError_t tomo_adj_n(const TomoArgs& a, const TIO* data, TIO* out, hipStream_t st) {
   constexpr int E = 8, N = N2 / 2;
   constexpr int C = tomo_cols_c<N2>();
-  const int64_t gcount = (int64_t)a.nz * N2 * N2;
-  int64_t zg = (gcount + 255) / 256;
-  hipLaunchKernelGGL(tomo_zero_k, dim3((unsigned)(zg > 65536 ? 65536 : zg)), dim3(256), 0, st, a.G, gcount);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
   constexpr int EL = N >= 8 ? 8 : N;
   constexpr int RL = 256 / (N / EL);
   const int64_t lines = (int64_t)a.ntheta * a.nz;
-  hipLaunchKernelGGL((tomo_scatter_k<N, EL, TIO>), dim3((unsigned)((lines + RL - 1) / RL)), dim3(256), 0, st, a,
+  hipLaunchKernelGGL((tomo_linefft_k<N, EL, TIO>), dim3((unsigned)((lines + RL - 1) / RL)), dim3(256), 0, st, a,
                      data);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int64_t n4 = (int64_t)N2 * N2;
+  const dim3 gg((unsigned)((n4 + 255) / 256), (unsigned)((a.nz + kZB - 1) / kZB));
+  if (a.nz % kZB == 0)
+    hipLaunchKernelGGL(tomo_grid_gather_k<true>, gg, dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(tomo_grid_gather_k<false>, gg, dim3(256), 0, st, a);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL((tomo_cols_adj_k<N2, E, C>), dim3((unsigned)(a.nz * (N2 / C))), dim3(C * (N2 / E)), 0, st, a);
   if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -377,10 +379,14 @@ struct htamd_tomo_plan {
   int m = 0;
   double mu = 0.0;
   int device = -1;
-  double* cs = nullptr;
   float* phi = nullptr;
   float2* tw2 = nullptr;
   float2* tw1 = nullptr;
+  int2* tb = nullptr;
+  float* tw = nullptr;
+  int* rp = nullptr;
+  int* ci = nullptr;
+  float* cw = nullptr;
 };
 
 extern "C" {
@@ -403,14 +409,60 @@ int htamd_tomo_plan_create(int64_t n, int64_t ntheta, const double* theta, const
     delete p;
     return fail(HTAMD_ERR_UNSUPPORTED, "tomo_plan_create: gridding half-width above 7");
   }
+  const int L = 2 * p->m + 1;
+  const int64_t S = ntheta * n, n2 = 2 * n;
+  if (S * L * L > (int64_t)INT32_MAX) {
+    delete p;
+    return fail(HTAMD_ERR_UNSUPPORTED, "tomo_plan_create: too many gridding taps (ntheta n (2m+1)^2 >= 2^31)");
+  }
   hipError_t e = hipGetDevice(&p->device);
-  // cos / sin of the angles as the reference rounds them (float32 theta)
-  std::vector<double> cs((size_t)(2 * ntheta));
+  // Gridding taps of every polar sample s = k n + t (cuda_kernels.py:17-51) in
+  // FP64: angles rounded to float32 as the reference stores them,
+  // x0 = (t - n/2) cos / n, y0 = -(t - n/2) sin / n, first tap floor(2n x0) - m,
+  // weights exp(-pi^2/mu d^2) (x) and pi/mu exp(-pi^2/mu d^2) (y).
+  std::vector<int2> tb((size_t)S);
+  std::vector<float> tw((size_t)(2 * L * S));
+  const double cy = M_PI / p->mu, cx = -M_PI * M_PI / p->mu, inv2n = 1.0 / (double)n2;
   for (int64_t k = 0; k < ntheta; ++k) {
     const double th = (double)(float)theta[k];
-    cs[(size_t)k] = std::cos(th);
-    cs[(size_t)(ntheta + k)] = std::sin(th);
+    const double c = std::cos(th), sn = std::sin(th);
+    for (int64_t t = 0; t < n; ++t) {
+      const int64_t si = k * n + t;
+      const double x0 = (double)(t - n / 2) / (double)n * c;
+      const double y0 = -(double)(t - n / 2) / (double)n * sn;
+      const int bx = (int)std::floor((double)n2 * x0) - p->m;
+      const int by = (int)std::floor((double)n2 * y0) - p->m;
+      tb[(size_t)si] = make_int2(bx, by);
+      for (int i = 0; i < L; ++i) {
+        const double dx = (bx + i) * inv2n - x0, dy = (by + i) * inv2n - y0;
+        tw[(size_t)(i * S + si)] = (float)std::exp(cx * dx * dx);
+        tw[(size_t)((L + i) * S + si)] = (float)(cy * std::exp(cx * dy * dy));
+      }
+    }
   }
+  // The same taps grouped by grid point (CSR, counting sort; within a row the
+  // samples appear in increasing order), for the adjoint's gather.
+  const int64_t ng = n2 * n2;
+  std::vector<int> rp((size_t)(ng + 1), 0);
+  auto wrap = [&](int64_t v) { return (int)((n + v + n2) % n2); };
+  for (int64_t si = 0; si < S; ++si)
+    for (int i1 = 0; i1 < L; ++i1)
+      for (int i0 = 0; i0 < L; ++i0)
+        ++rp[(size_t)(wrap(tb[(size_t)si].y + i1) * n2 + wrap(tb[(size_t)si].x + i0) + 1)];
+  for (int64_t g = 0; g < ng; ++g) rp[(size_t)(g + 1)] += rp[(size_t)g];
+  std::vector<int> fill(rp.begin(), rp.end() - 1);
+  std::vector<int> ci((size_t)(S * L * L));
+  std::vector<float> cw((size_t)(S * L * L));
+  for (int64_t si = 0; si < S; ++si)
+    for (int i1 = 0; i1 < L; ++i1) {
+      const float wy = tw[(size_t)((L + i1) * S + si)];
+      for (int i0 = 0; i0 < L; ++i0) {
+        const int64_t g = wrap(tb[(size_t)si].y + i1) * n2 + wrap(tb[(size_t)si].x + i0);
+        const int k = fill[(size_t)g]++;
+        ci[(size_t)k] = (int)si;
+        cw[(size_t)k] = wy * tw[(size_t)(i0 * S + si)];
+      }
+    }
   // deapodiser phi (tomo.py:19-21, 46-47) with the mask folded in
   std::vector<float> phi((size_t)(n * n));
   for (int64_t i = 0; i < n; ++i) {
@@ -422,7 +474,7 @@ int htamd_tomo_plan_create(int64_t n, int64_t ntheta, const double* theta, const
       phi[(size_t)(i * n + j)] = (float)v;
     }
   }
-  auto tw = [](int64_t N) {
+  auto twid = [](int64_t N) {
     std::vector<float2> t((size_t)N);
     for (int64_t k = 0; k < N; ++k) {
       const double a = -2.0 * M_PI * (double)k / (double)N;
@@ -435,8 +487,10 @@ int htamd_tomo_plan_create(int64_t n, int64_t ntheta, const double* theta, const
     if (r == hipSuccess) r = hipMemcpy(*dst, src.data(), src.size() * sizeof(src[0]), hipMemcpyHostToDevice);
     return r;
   };
-  if (e != hipSuccess || (e = up(&p->cs, cs)) != hipSuccess || (e = up(&p->phi, phi)) != hipSuccess ||
-      (e = up(&p->tw2, tw(2 * n))) != hipSuccess || (e = up(&p->tw1, tw(n))) != hipSuccess) {
+  if (e != hipSuccess || (e = up(&p->phi, phi)) != hipSuccess || (e = up(&p->tw2, twid(2 * n))) != hipSuccess ||
+      (e = up(&p->tw1, twid(n))) != hipSuccess || (e = up(&p->tb, tb)) != hipSuccess ||
+      (e = up(&p->tw, tw)) != hipSuccess || (e = up(&p->rp, rp)) != hipSuccess || (e = up(&p->ci, ci)) != hipSuccess ||
+      (e = up(&p->cw, cw)) != hipSuccess) {
     htamd_tomo_plan_destroy(p);
     return hip_fail(e, "tomo_plan_create");
   }
@@ -446,17 +500,21 @@ int htamd_tomo_plan_create(int64_t n, int64_t ntheta, const double* theta, const
 
 int htamd_tomo_plan_destroy(htamd_tomo_plan* p) {
   if (!p) return HTAMD_OK;
-  (void)hipFree(p->cs);
   (void)hipFree(p->phi);
   (void)hipFree(p->tw2);
   (void)hipFree(p->tw1);
+  (void)hipFree(p->tb);
+  (void)hipFree(p->tw);
+  (void)hipFree(p->rp);
+  (void)hipFree(p->ci);
+  (void)hipFree(p->cw);
   delete p;
   return HTAMD_OK;
 }
 
 int htamd_tomo_workspace_size(const htamd_tomo_plan* p, int64_t nz, int64_t* bytes) {
   if (!p || !bytes || nz < 0) return fail(HTAMD_ERR_ARG, "tomo_workspace_size: bad arguments");
-  *bytes = nz * (p->n * 2 * p->n + 4 * p->n * p->n) * 8;
+  *bytes = nz * (p->n * 2 * p->n + 4 * p->n * p->n) * 8 + p->ntheta * p->n * nz * 8;
   return HTAMD_OK;
 }
 
@@ -478,13 +536,17 @@ static int tomo_run(const htamd_tomo_plan* p, bool adjoint, const void* in, int 
   a.nz = (int)nz;
   a.ntheta = (int)p->ntheta;
   a.m = p->m;
-  a.mu = p->mu;
-  a.cs = p->cs;
   a.phi = p->phi;
+  a.tb = p->tb;
+  a.tw = p->tw;
+  a.rp = p->rp;
+  a.ci = p->ci;
+  a.cw = p->cw;
   a.tw2 = p->tw2;
   a.tw1 = p->tw1;
   a.W = (float2*)ws;
   a.G = (float2*)ws + nz * p->n * 2 * p->n;
+  a.Sh = a.G + nz * 4 * p->n * p->n;
   hipError_t e = is_complex ? tomo_dispatch<float2>(adjoint, a, in, out, (hipStream_t)stream)
                             : tomo_dispatch<float>(adjoint, a, in, out, (hipStream_t)stream);
   return e == hipSuccess ? HTAMD_OK : hip_fail(e, what);

@@ -235,8 +235,9 @@ typedef struct htamd_tomo_plan htamd_tomo_plan;
 int htamd_tomo_plan_create(int64_t n, int64_t ntheta, const double* theta, const double* mask,
                            htamd_tomo_plan** plan);
 int htamd_tomo_plan_destroy(htamd_tomo_plan* plan);
-/* Device scratch for nz slices: nz * 6 n^2 complex64 (the 2n x 2n frequency
- * grid plus one half-transformed n x 2n stage). */
+/* Device scratch for nz slices: nz (6 n^2 + ntheta n) complex64 (the 2n x 2n
+ * frequency grid, one half-transformed n x 2n stage and the transformed
+ * sinogram of the adjoint). */
 int htamd_tomo_workspace_size(const htamd_tomo_plan* plan, int64_t nz, int64_t* bytes);
 /* Replaces Tomo.R (src/holotomocupy/tomo.py:56-86) with the gather kernel
  * (src/holotomocupy/cuda_kernels.py:3-76, dir 0): obj [nz, n, n] -> out
@@ -245,8 +246,9 @@ int htamd_tomo_workspace_size(const htamd_tomo_plan* plan, int64_t nz, int64_t* 
 int htamd_tomo_R(const htamd_tomo_plan* plan, const void* obj, int is_complex, void* out, int64_t nz,
                  void* workspace, int64_t workspace_bytes, void* stream);
 /* Replaces Tomo.RT (src/holotomocupy/tomo.py:88-116) with the scatter of
- * cuda_kernels.py:3-76 (dir 1, float atomics): data [ntheta, nz, n] ->
- * out [nz, n, n]; dtypes as for htamd_tomo_R. */
+ * cuda_kernels.py:3-76 (dir 1) computed as a deterministic gather over the
+ * transposed taps: data [ntheta, nz, n] -> out [nz, n, n]; dtypes as for
+ * htamd_tomo_R. */
 int htamd_tomo_RT(const htamd_tomo_plan* plan, const void* data, int is_complex, void* out, int64_t nz,
                   void* workspace, int64_t workspace_bytes, void* stream);
 

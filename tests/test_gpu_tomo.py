@@ -41,6 +41,7 @@ CASES = [  # n, nz, ntheta, mask_r, nz used
     (32, 2, 24, 0.9, 2),
     (64, 4, 48, 0, 3),
     (128, 2, 30, 0.8, 2),
+    (32, 32, 20, 0, 32),  # whole 16-slice blocks in the adjoint's gather
 ]
 
 
