@@ -566,18 +566,27 @@ static BhLayout bh_layout(const htamd_prop_plan* pp, const htamd_shift_plan* sp,
 
 // LDS of one workgroup: the statically sized buffers plus the staged object
 // rows (one row of npsi values per detector row, two in the adjoint pass);
-// rows are staged only when the total stays within 64 KiB.
-constexpr int64_t kRowLdsBudget = 65536;
+// rows are staged when the staged part fits 64 KiB and the total the 160 KiB
+// of a CU.
+constexpr int64_t kRowDynBudget = 65536;
+constexpr int64_t kRowLdsBudget = 160 * 1024;
+
+// Row-pass elements per thread (one row per 256 threads at N = 2048; 16
+// elements with radix-16 stages measured slower: fewer waves per CU).
+template <int N>
+constexpr int bh_row_e() {
+  return 8;
+}
 
 template <int N, typename TO>
 static hipError_t launch_rows(bool adjoint, const BhArgs& a, hipStream_t st) {
-  constexpr int E = 8;
+  constexpr int E = bh_row_e<N>();
   constexpr int ROWS = kRowThreads / (N / E);
   const int64_t units = (int64_t)a.ndist * a.nz;
   const int64_t grid = (units + ROWS - 1) / ROWS;
   const int64_t stat = (int64_t)ROWS * (fft_buf<N>() + (adjoint ? N : 0)) * 8 + 64;
   const int64_t dyn = (int64_t)ROWS * a.npsi * (int64_t)sizeof(TO) * (adjoint ? 2 : 1);
-  const bool staged = stat + dyn <= kRowLdsBudget;
+  const bool staged = dyn <= kRowDynBudget && stat + dyn <= kRowLdsBudget;
   if (adjoint) {
     if (staged)
       hipLaunchKernelGGL((bh_adj_rows_k<N, E, TO, true>), dim3((unsigned)grid), dim3(kRowThreads), (size_t)dyn,
