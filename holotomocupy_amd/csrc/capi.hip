@@ -37,6 +37,13 @@ static std::vector<float2> make_twiddles(int64_t N) {
   return tw;
 }
 
+hipError_t upload_twiddles(float2** dst, int64_t N) {
+  const std::vector<float2> tw = make_twiddles(N);
+  hipError_t e = hipMalloc((void**)dst, tw.size() * sizeof(float2));
+  if (e == hipSuccess) e = hipMemcpy(*dst, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice);
+  return e;
+}
+
 // Pair table of the 1-D operator C^-1 diag(l) C of length N (see fresnel.hip),
 // l_k = exp(sign * i * coef * k^2), coef = pi*lambda*z/(4 N^2 vs^2), FP64.
 static void fill_pair_table(float2* tab, int64_t N, double coef, double sign) {

@@ -82,11 +82,22 @@ int htamd_shift_plan_create(int64_t n, int64_t npsi, int64_t nz, int64_t nzpsi, 
   p->is_complex = is_complex ? 1 : 0;
   prefilter_taps(npsi, p->taps.wx);
   prefilter_taps(nzpsi, p->taps.wy);
+  if (fft_size_supported(npsi) && fft_size_supported(nzpsi)) {
+    hipError_t e = upload_twiddles(&p->tw_x, npsi);
+    if (e == hipSuccess) e = upload_twiddles(&p->tw_y, nzpsi);
+    if (e != hipSuccess) {
+      htamd_shift_plan_destroy(p);
+      return hip_fail(e, "shift_plan_create");
+    }
+  }
   *plan = p;
   return HTAMD_OK;
 }
 
 int htamd_shift_plan_destroy(htamd_shift_plan* p) {
+  if (!p) return HTAMD_OK;
+  (void)hipFree(p->tw_x);
+  (void)hipFree(p->tw_y);
   delete p;
   return HTAMD_OK;
 }

@@ -12,6 +12,8 @@ void set_error(const std::string& msg);
 // set the message and return `code` / HTAMD_ERR_HIP
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* where);
+// tw[m] = exp(-2 pi i m / N), m < N, generated in FP64, uploaded to *dst
+hipError_t upload_twiddles(float2** dst, int64_t N);
 
 // ---- Fresnel propagator (fresnel.hip) -------------------------------------
 // Smallest / largest power-of-two transform length handled by the LDS FFT.
@@ -46,4 +48,8 @@ struct htamd_shift_plan {
   int64_t n = 0, npsi = 0, nz = 0, nzpsi = 0;
   int is_complex = 1;
   htamd::PrefilterTaps taps{};
+  // FFT twiddles of the object grid (coeffback), allocated on the current
+  // device when npsi and nzpsi are transform sizes of the LDS FFT, else NULL
+  float2* tw_x = nullptr;
+  float2* tw_y = nullptr;
 };

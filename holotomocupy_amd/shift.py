@@ -161,3 +161,39 @@ class Shift:
         _lib.call("htamd_bspline_d1_adj", self._plan, _lib.ptr(c), _lib.ptr(g), rp, rs, mp, ms,
                   _lib.ptr(out1), _lib.ptr(out2), _lib.ptr(ws), nbytes.value, B, _lib.stream_ptr())
         return [out1, out2]
+
+    # ------------------------------------------------------------------ back-shift (initial guess)
+    def Sback(self, spsi, r, m):
+        """Magnified back-shift of detector images onto the object grid with the
+        stretched spline phi(t/m) (shift.py:138-149, sback kernel
+        cuda_kernels.py:294-368); complex64 out, as in the reference."""
+        g = _lib.as_device(spsi, torch.complex64, self.device)
+        g = g[None] if g.ndim == 2 else g
+        if g.ndim != 3 or tuple(g.shape[1:]) != (self.nz, self.n):
+            raise ValueError(f"expected [B, {self.nz}, {self.n}], got {tuple(spsi.shape)}")
+        B = g.shape[0]
+        r, rp, rs = self._shifts(r, B)
+        m, mp, ms = self._mags(m, B)
+        out = torch.empty((B, self.nzpsi, self.npsi), dtype=torch.complex64, device=self.device)
+        _lib.call("htamd_bspline_sback", self._plan, _lib.ptr(g), rp, rs, mp, ms, _lib.ptr(out), B,
+                  _lib.stream_ptr())
+        return out
+
+    def coeffback(self, psi, m):
+        """IFFT2(FFT2(psi) / (D_y (x) D_x)) with D(f) = phi(0) + 2 sum_{k=1..4}
+        phi(k/m) cos(2 pi k f) (shift.py:84-96); m is one magnification."""
+        mv = np.asarray(m.cpu() if isinstance(m, torch.Tensor) else m, dtype=np.float64).reshape(-1)
+        if mv.size != 1:
+            raise ValueError("coeffback takes one magnification")
+        x = _lib.as_device(psi, torch.complex64, self.device)
+        x = x[None] if x.ndim == 2 else x
+        if x.ndim != 3 or tuple(x.shape[1:]) != (self.nzpsi, self.npsi):
+            raise ValueError(f"expected [B, {self.nzpsi}, {self.npsi}], got {tuple(psi.shape)}")
+        out = torch.empty_like(x)
+        _lib.call("htamd_bspline_coeffback", self._plan, _lib.ptr(x), _lib.ptr(out), x.shape[0], float(mv[0]),
+                  _lib.stream_ptr())
+        return out
+
+    def curlySback(self, psi, r, m):
+        """coeffback(Sback(psi, r, m), m)  (shift.py:151-153)."""
+        return self.coeffback(self.Sback(psi, r, m), m)

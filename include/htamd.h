@@ -131,6 +131,21 @@ int htamd_bspline_d1_adj(const htamd_shift_plan* plan, const void* c, const void
                          int64_t r_stride, const float* m, int64_t m_stride, void* out1, float* out2,
                          void* workspace, int64_t workspace_bytes, int64_t batch, void* stream);
 
+/* Magnified back-shift of the Paganin initial guess (complex64 only):
+ * c_out[batch, nzpsi, npsi] = Sback(g, r, m), g [batch, nz, n], the stretched
+ * spline phi(t/m) of support |t| < 2m, computed as a deterministic gather.
+ * Replaces Shift.Sback (src/holotomocupy/shift.py:138-149) and the sback kernel
+ * (src/holotomocupy/cuda_kernels.py:294-368). */
+int htamd_bspline_sback(const htamd_shift_plan* plan, const void* g, const float* r, int64_t r_stride,
+                        const float* m, int64_t m_stride, void* c_out, int64_t batch, void* stream);
+
+/* out = IFFT2(FFT2(in) / (D_y (x) D_x)), D(f) = phi(0) + 2 sum_{k=1..4} phi(k/m)
+ * cos(2 pi k f), in / out [batch, nzpsi, npsi] complex64 (may alias); npsi and
+ * nzpsi powers of two in [8, 2048].  Replaces Shift.coeffback
+ * (src/holotomocupy/shift.py:84-96). */
+int htamd_bspline_coeffback(const htamd_shift_plan* plan, const void* in, void* out, int64_t batch, double m,
+                            void* stream);
+
 /* ---- pointwise cascade stages and reductions -------------------------------
  * Complex arrays are complex64; per-angle fields are [batch, ndist, npix]
  * (npix = nz*n), probe-like arrays [ndist, npix]. */

@@ -1,6 +1,8 @@
 """Oracle: cubic B-spline shift family (TEST INFRASTRUCTURE ONLY).
 
-Restates src/holotomocupy/shift.py:18-221 and the device kernels
+Restates src/holotomocupy/shift.py:18-221 (with the back-shift Sback /
+coeffback of shift.py:84-153 and its sback kernel, cuda_kernels.py:294-368)
+and the device kernels
 src/holotomocupy/cuda_kernels.py:131-290 (s / sf), 377-542 (d2s / d2sf),
 547-687 (ds / dsf) and 690-843 (dsadj / dsadjf).
 
@@ -16,7 +18,7 @@ this module builds those matrices and applies them, which is the same sum in
 another order.  The adjoints use the transposed matrices -- the reference's
 atomic scatter (dir = 1) computes exactly Wy^T g Wx.
 
-Pins (tests/test_oracle_shift.py):
+Pins (tests/test_oracle_shift.py; the back-shift in tests/test_oracle_sback.py):
   * S(coeff(psi), r, 1) equals scipy.ndimage.shift(psi, -r, order=3,
     mode='grid-wrap') on compactly supported inputs;
   * S(c, r, m) equals scipy.ndimage.map_coordinates(36 c, ..., order=3,
@@ -201,3 +203,64 @@ class Shift:
             out2[b, 0] = np.sum(np.real(g) * np.real(dt1) + np.imag(g) * np.imag(dt1))
             out2[b, 1] = np.sum(np.real(g) * np.real(dt2) + np.imag(g) * np.imag(dt2))
         return [np.stack(out1).astype(self.dtype), out2]
+
+
+# ---------------------------------------------------------------- back-shift (initial guess)
+def phi_back(t, m):
+    """phi(t / m) of the sback kernel (cuda_kernels.py:297-304)."""
+    return phi(np.asarray(t, dtype=np.float64) / m)
+
+
+def back_axis_matrix(npix, npsi, coord, m):
+    """Sparse [npix, npsi] matrix of one axis of the sback kernel
+    (cuda_kernels.py:316-356): row t holds phi((d - j)/m) at columns
+    floor(coord[t]) + j for j in [ceil(d - 2m), (int)(d + 2m)) (the upper
+    bound exclusive, as the kernel's loop has it), in range."""
+    i0 = np.floor(coord)
+    d = coord - i0
+    span = 2.0 * m
+    rows, cols, vals = [], [], []
+    for t in range(npix):
+        for j in range(int(np.ceil(d[t] - span)), int(np.floor(d[t] + span))):
+            idx = int(i0[t]) + j
+            if 0 <= idx < npsi:
+                rows.append(t)
+                cols.append(idx)
+                vals.append(phi_back(d[t] - j, m))
+    return sp.csr_matrix((np.asarray(vals, dtype=np.float64), (rows, cols)), shape=(npix, npsi))
+
+
+def sback_mats(n, npsi, nz, nzpsi, r, m):
+    x, y = coords(n, nz, npsi, nzpsi, r, m)
+    m = np.broadcast_to(np.asarray(m, dtype=np.float64).reshape(-1), (x.shape[0],))
+    return ([back_axis_matrix(nz, nzpsi, y[b], m[b]) for b in range(x.shape[0])],
+            [back_axis_matrix(n, npsi, x[b], m[b]) for b in range(x.shape[0])])
+
+
+def sback(g, r, m, n, npsi, nz, nzpsi):
+    """Shift.Sback (shift.py:138-149): the sback scatter (dir 1) as Wy^T g Wx."""
+    g = np.asarray(g, dtype=np.complex128)
+    Wy, Wx = sback_mats(n, npsi, nz, nzpsi, r, m)
+    return np.stack([Wy[b].T @ g[b] @ Wx[b] for b in range(g.shape[0])])
+
+
+def sback_forward(c, r, m, n, npsi, nz, nzpsi):
+    """dir 0 of the sback kernel (the gather whose transpose Sback is)."""
+    c = np.asarray(c, dtype=np.complex128)
+    Wy, Wx = sback_mats(n, npsi, nz, nzpsi, r, m)
+    return np.stack([Wy[b] @ c[b] @ Wx[b].T for b in range(c.shape[0])])
+
+
+def coeffback_filter(N, m):
+    """D(f) = phi(0) + 2 sum_{k=1..4} phi(k/m) cos(2 pi k x) on
+    x = linspace(-1/2, 1/2 - 1/N, N), fftshifted (shift.py:84-96)."""
+    x = np.linspace(-0.5, 0.5 - 1.0 / N, N)
+    d = phi(0.0) + sum(2 * phi(k / m) * np.cos(2 * np.pi * k * x) for k in range(1, 5))
+    return np.fft.fftshift(d)
+
+
+def coeffback(psi, m):
+    """Shift.coeffback: IFFT2(FFT2(psi) / (D_y (x) D_x))."""
+    psi = np.asarray(psi, dtype=np.complex128)
+    H = 1.0 / np.outer(coeffback_filter(psi.shape[-2], m), coeffback_filter(psi.shape[-1], m))
+    return sfft.ifft2(sfft.fft2(psi, axes=(-2, -1), workers=WORKERS) * H, axes=(-2, -1), workers=WORKERS)
