@@ -259,9 +259,6 @@ __global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_f
   fft_load_twiddles<N, E>(twr, t, a.twy);
   float2* base = a.T + f * (int64_t)N * a.n + x0;
   const float* dbase = a.data + f * (int64_t)N * a.n + x0;
-  float dv[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) dv[e] = dbase[dct_nat<N>(t + e * T) * a.n + cc];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int idx = threadIdx.x + e * NT;
@@ -275,6 +272,11 @@ __global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_f
   double obj = 0.0;
   int tr = t;
   asm volatile("" : "+v"(tr));
+  // sqrt(data) of the column, read after the first transform (holding it
+  // across the transform costs registers the wide tiles do not have)
+  float dv[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) dv[e] = dbase[dct_nat<N>(tr + e * T) * a.n + cc];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int p = fpad(tr + e * T);
@@ -514,15 +516,27 @@ struct BhLayout {
   int64_t c = 0, T = 0, dpsi = 0, part_r = 0, part_obj = 0, total = 0;
 };
 
+// Column pass of the fused gradient: 16 elements per thread at N = 2048 (two
+// columns per 256-thread workgroup), else 8.
 template <int N>
-constexpr int bh_col_e();
+constexpr int bh_col_e() {
+  return N >= 2048 ? 16 : 8;
+}
+
+// Column tile width C of K2 (C * 8 contiguous bytes per row): bounded by 256
+// threads and the 160 KiB of LDS of a CU.  The pass does two operators per
+// load and store, so the strided access matters less than for a single
+// operator; at 2048 (16 elements per thread) C = 2 and C = 4 measure the
+// same, 32 elements with C = 8 and 8 elements with C = 4 are slower.
+template <int NZ, int C>
+constexpr bool col_tile_fits() {
+  return C * (NZ / bh_col_e<NZ>()) <= 256 && (int64_t)C * col_stride<NZ, C>() * 8 + 4096 <= 160 * 1024;
+}
 
 template <int NZ>
 static int col_tiles_c(int64_t n) {
-  constexpr int TT = NZ / bh_col_e<NZ>();
-  constexpr int CT = 256 / TT, CL = 8192 / NZ;
-  constexpr int CM0 = CT < CL ? CT : CL;
-  constexpr int CM = CM0 > 16 ? 16 : (CM0 < 1 ? 1 : CM0);
+  constexpr int CM = col_tile_fits<NZ, 16>() ? 16 : col_tile_fits<NZ, 8>() ? 8 : col_tile_fits<NZ, 4>() ? 4
+                     : col_tile_fits<NZ, 2>() ? 2 : 1;
   for (int c = CM; c > 1; c >>= 1)
     if (n % c == 0) return c;
   return 1;
@@ -619,12 +633,6 @@ static hipError_t rows_dispatch(bool adjoint, const BhArgs& a, hipStream_t st) {
   }
 }
 
-// Column pass of the fused gradient: 16 elements per thread at N = 2048 (two
-// columns per 256-thread workgroup), else 8.
-template <int N>
-constexpr int bh_col_e() {
-  return N >= 2048 ? 16 : 8;
-}
 
 template <int NZ>
 static hipError_t launch_cols(const BhArgs& a, hipStream_t st) {
@@ -635,9 +643,11 @@ static hipError_t launch_cols(const BhArgs& a, hipStream_t st) {
   switch (C) {
 #define HTAMD_BHC(CC)                                                                                 \
   case CC:                                                                                            \
-    if constexpr (CC * (NZ / E) <= 256 && CC * NZ <= 8192)                                           \
+    if constexpr (col_tile_fits<NZ, CC>())                                                            \
       hipLaunchKernelGGL((bh_cols_k<NZ, E, CC>), dim3((unsigned)grid), dim3(CC * (NZ / E)), 0, st, a, \
                          tpf);                                                                        \
+    else                                                                                              \
+      return hipErrorInvalidValue;                                                                    \
     break;
     HTAMD_BHC(1) HTAMD_BHC(2) HTAMD_BHC(4) HTAMD_BHC(8) HTAMD_BHC(16)
 #undef HTAMD_BHC

@@ -31,6 +31,8 @@ __device__ __forceinline__ void dct_pair_map(float2* s, int t, const float2* __r
       s[pk] = cadd(cmul(a, v1), cmul(b, v2));
       s[pm] = cadd(cmul(c, v1), cmul(a, v2));
     }
+    // keep the compiler from hoisting all 3 E/2 table loads at once (registers)
+    if (q % 4 == 3) asm volatile("" ::: "memory");
   }
   __syncthreads();
 }

@@ -98,30 +98,38 @@ def test_strided_shift_views_and_2d(gpu):
 
 
 def test_adjoint_identities_large(gpu):
-    """C2-sized shift: 256^2, 32 angles, shifts U[-5, 5], adjoint dot tests."""
+    """C2-sized shift: 256^2, 32 angles, shifts U[-5, 5], adjoint dot tests.
+    Each identity <A x, y> = <x, A* y> is checked relative to ||A x|| ||y||
+    (the Cauchy-Schwarz scale): for random x, y the inner product itself can
+    be far smaller than that, and FP32 rounding is relative to the scale."""
     print("[case] adjoint 256^2 x 32", flush=True)
     from holotomocupy_amd import Shift
     rng = np.random.default_rng(1002)
+    gen = torch.Generator(device="cuda").manual_seed(1002)
     B, n = 32, 256
     g = Shift(n, n, n, n, "complex64")
     r = dev(rng.uniform(-5, 5, (B, 2)).astype(np.float32))
     m = torch.ones(B, device="cuda")
-    c = torch.randn(B, n, n, dtype=torch.complex64, device="cuda")
-    s = torch.randn(B, n, n, dtype=torch.complex64, device="cuda")
-    lhs = torch.vdot(g.S(c, r, m).flatten().to(torch.complex128), s.flatten().to(torch.complex128))
-    rhs = torch.vdot(c.flatten().to(torch.complex128), g.Sadj(s, r, m).flatten().to(torch.complex128))
-    assert abs(lhs - rhs) / abs(lhs) < REL_TOL
-    lhs = torch.vdot(g.coeff(c).flatten().to(torch.complex128), s.flatten().to(torch.complex128))
-    rhs = torch.vdot(c.flatten().to(torch.complex128), g.coeff(s).flatten().to(torch.complex128))
-    assert abs(lhs - rhs) / abs(lhs) < REL_TOL
+    c = torch.randn(B, n, n, dtype=torch.complex64, device="cuda", generator=gen)
+    s = torch.randn(B, n, n, dtype=torch.complex64, device="cuda", generator=gen)
+    d128 = lambda t: t.flatten().to(torch.complex128)  # noqa: E731
+    Sc = g.S(c, r, m)
+    lhs = torch.vdot(d128(Sc), d128(s))
+    rhs = torch.vdot(d128(c), d128(g.Sadj(s, r, m)))
+    assert abs(lhs - rhs) / (torch.linalg.norm(d128(Sc)) * torch.linalg.norm(d128(s))) < 1e-6
+    Cc = g.coeff(c)
+    lhs = torch.vdot(d128(Cc), d128(s))
+    rhs = torch.vdot(d128(c), d128(g.coeff(s)))
+    assert abs(lhs - rhs) / (torch.linalg.norm(d128(Cc)) * torch.linalg.norm(d128(s))) < 1e-6
     # dSadj identity: <dS(c1, dr), g> = <c1, out1> + dr . out2
-    c1 = torch.randn_like(c)
+    c1 = torch.randn(c.shape, dtype=torch.complex64, device="cuda", generator=gen)
     dr = dev(rng.uniform(-1, 1, (B, 2)).astype(np.float32))
     cc = g.coeff(c)
-    lhs = (g.dcurlySc(cc, r, m, c1, dr).to(torch.complex128).conj() * s.to(torch.complex128)).real.sum()
+    dS = g.dcurlySc(cc, r, m, c1, dr)
+    lhs = (d128(dS).conj() * d128(s)).real.sum()
     o1, o2 = g.dcurlySadjc(cc, r, m, s)
-    rhs = (c1.to(torch.complex128).conj() * o1.to(torch.complex128)).real.sum() + (dr.double() * o2.double()).sum()
-    assert abs(lhs - rhs) / abs(lhs) < REL_TOL
+    rhs = (d128(c1).conj() * d128(o1)).real.sum() + (dr.double() * o2.double()).sum()
+    assert abs(lhs - rhs) / (torch.linalg.norm(d128(dS)) * torch.linalg.norm(d128(s))) < 1e-6
 
 
 def test_deterministic(gpu):
