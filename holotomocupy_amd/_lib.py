@@ -42,6 +42,9 @@ SIGNATURES = {
     "htamd_blas1_workspace_size": [ctypes.POINTER(c_i64)],
     "htamd_redot": [c_vp, c_vp, c_i64, c_vp, c_int, c_vp, c_i64, c_vp],
     "htamd_axpby_redot": [c_vp, c_vp, ctypes.c_float, ctypes.c_float, c_i64, c_vp, c_int, c_vp, c_i64, c_vp],
+    "htamd_axpby_dc": [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp],
+    "htamd_axpby_redot_dc": [c_vp, c_vp, c_vp, c_i64, c_vp, c_dbl, c_int, c_vp, c_i64, c_vp],
+    "htamd_laplacian3d": [c_vp, c_i64, c_i64, c_i64, c_int, c_i64, c_i64, c_vp, ctypes.c_float, c_vp, c_vp],
     "htamd_tomo_plan_create": [c_i64, c_i64, c_vp, c_vp, ctypes.POINTER(c_vp)],
     "htamd_tomo_plan_destroy": [c_vp],
     "htamd_tomo_workspace_size": [c_vp, c_i64, ctypes.POINTER(c_i64)],

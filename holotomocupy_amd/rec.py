@@ -23,6 +23,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from .bhloop import BHLoop
 from .propagation import Propagation
 from .shift import Shift
 
@@ -41,14 +42,31 @@ def physics(energy, z1, focustodetectordistance, detector_pixelsize):
                            magnifications=magnifications, norm_magnifications=norm_magnifications)
 
 
-class Rec:
-    """Cascade part of rec_mpi.Rec (rec_mpi.py:21-103 construction, 250-813 cascade).
+class Rec(BHLoop):
+    """rec_mpi.Rec on one GPU (rec_mpi.py:21-103 construction, 138-325 BH loop,
+    250-813 cascade; the loop itself is holotomocupy_amd.bhloop.BHLoop).
 
     ``args`` carries the reference's fields: energy, detector_pixelsize,
     focustodetectordistance, z1, ndist, ntheta, nz, n, nzobj, nobj, obj_dtype,
-    rho, and optionally lam_prbfit (default 0) and shrink (per-angle,
-    per-distance magnification change, default 0).
+    rho, and optionally lam_prbfit (default 0), lam_laplacian (default 0),
+    shrink (per-angle, per-distance magnification change, default 0), nchunk
+    (angles per launch, default ntheta), and for the BH loop theta, mask_r,
+    niter, start_iter, err_step, vis_step, path_out.  vars / grads / etas /
+    data / ref are allocated on the device when first used (alloc_arrays).
     """
+
+    _LAZY = ("vars", "grads", "etas", "data", "ref", "table")
+
+    def __getattr__(self, name):
+        # only reached when normal lookup fails: allocate the loop's arrays once
+        if name in Rec._LAZY and not self.__dict__.get("_allocating", False):
+            self.__dict__["_allocating"] = True
+            try:
+                self.alloc_arrays()
+            finally:
+                self.__dict__["_allocating"] = False
+            return self.__dict__[name]
+        raise AttributeError(name)
 
     def __init__(self, args):
         _lib.require_gpu()
@@ -300,6 +318,10 @@ class Rec:
 
     def hessian_cascade(self, vars, grads, etas, data=None, st=0):
         """BH bilinear form of the data term (rec_mpi.py:250-303), as a float."""
+        return float(self._hessian_cascade_dev(vars, grads, etas, data, st))
+
+    def _hessian_cascade_dev(self, vars, grads, etas, data=None, st=0):
+        """hessian_cascade as a float64 device tensor [1] (no host sync)."""
         data = self.data if data is None else data
         self._chunk(st, vars["proj"].shape[0])
         x = [vars["prb"], vars["proj"], vars["pos"]]
@@ -312,16 +334,19 @@ class Rec:
             fx, y = self.dF[i](x, y)
             z = y if y_is_z else self.dF[i](x, z, return_x=False)
             x = fx
-        return float(self.d2F_dF0(x, y, z, w, data))
+        return self.d2F_dF0(x, y, z, w, data).reshape(1)
 
     def objective(self, vars, data=None, st=0):
         """Data term of Rec.min (rec_mpi.py:815-835) for the given angles."""
+        return float(self._objective_dev(vars, data, st))
+
+    def _objective_dev(self, vars, data=None, st=0):
         data = self.data if data is None else data
         self._chunk(st, vars["proj"].shape[0])
         y = [vars["prb"], vars["proj"], vars["pos"]]
         for i in range(1, len(self.F))[::-1]:
             y = self.F[i](y)
-        return float(self.F0(y, data))
+        return self.F0(y, data).reshape(1)
 
     # ------------------------------------------------------------------ probe fit
     def gradient_prbfit(self, grad_prb, prb):
@@ -375,6 +400,11 @@ class Rec:
         over the slices (rec_mpi.py:498-507); host or device arrays."""
         cl = self._chunking()
         tomo = self._tomo()
+        if isinstance(obj, torch.Tensor) and obj.is_cuda and isinstance(out, torch.Tensor) and out.is_cuda:
+            for z0 in range(0, obj.shape[0], tomo._nz):
+                z1 = min(obj.shape[0], z0 + tomo._nz)
+                out[:, z0:z1] = tomo.R(obj[z0:z1])
+            return out
 
         @cl.gpu_batch(axis_out=1, axis_inp=0, nout=1)
         def _fwd_tomo(self, out, obj):
@@ -388,6 +418,12 @@ class Rec:
         chunk (rec_mpi.py:355-362)."""
         cl = self._chunking()
         tomo = self._tomo()
+        if isinstance(gradu, torch.Tensor) and gradu.is_cuda and isinstance(gradproj, torch.Tensor) \
+                and gradproj.is_cuda:
+            for z0 in range(0, gradu.shape[0], tomo._nz):
+                z1 = min(gradu.shape[0], z0 + tomo._nz)
+                gradu[z0:z1] = tomo.RT(gradproj[:, z0:z1].contiguous())
+            return
 
         @cl.gpu_batch(axis_out=0, axis_inp=1, nout=1)
         def _gF4(self, gradu, gradproj):

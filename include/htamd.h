@@ -239,6 +239,25 @@ int htamd_redot(const void* x, const void* y, int64_t n, double* result, int acc
  * then result (+)= Re<y, x>, in one pass. */
 int htamd_axpby_redot(void* x, const void* y, float a, float b, int64_t n, double* result, int accumulate,
                       void* workspace, int64_t workspace_bytes, void* stream);
+/* Device-coefficient forms for a BH loop that keeps its step sizes on the device (rec_mpi.py:188-222
+ * compute alpha and beta on the host after a synchronising .get()): ab points to two device doubles
+ * (a, b), rounded to float as the host forms round their float arguments.
+ * htamd_axpby_dc replaces linear_batch (chunking.py:249-262); htamd_axpby_redot_dc replaces
+ * linear_redot_batch (chunking.py:264-278) and adds scale * Re<y, x> to result (with accumulate) so
+ * that the loop's weighted sum  top -= Re<g_v, eta_v> / rho_v^2  (rec_mpi.py:198-205) stays on the
+ * device. */
+int htamd_axpby_dc(const void* x, const void* y, const double* ab, void* out, int64_t n, void* stream);
+int htamd_axpby_redot_dc(void* x, const void* y, const double* ab, int64_t n, double* result, double scale,
+                         int accumulate, void* workspace, int64_t workspace_bytes, void* stream);
+
+/* ---- 3-D Laplacian of the regulariser -----------------------------------------
+ * Replaces `_lap` and the stencil steps of gradient_laplacian / hessian_laplacian / _lap_energy_local
+ * (src/holotomocupy/rec_mpi.py:418-494): out[k] = add[k] + scale * lap(in)[z0 + k] for
+ * k < nz_out, with the 7-point stencil periodic in y and x and zero outside slices [0, nz_in) in z.
+ * in: [nz_in, ny, nx] float32 (or complex64 if is_complex); add may be NULL or alias out; out must
+ * not alias in. */
+int htamd_laplacian3d(const void* in, int64_t nz_in, int64_t ny, int64_t nx, int is_complex, int64_t z0,
+                      int64_t nz_out, const void* add, float scale, void* out, void* stream);
 
 /* ---- USFFT Radon transform ---------------------------------------------------
  * Geometry of Tomo(n, nz, theta, mask_r) (src/holotomocupy/tomo.py:12-54): slices
