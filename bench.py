@@ -36,10 +36,10 @@ def parse():
     p.add_argument("--path", choices=["auto", "plain", "fused", "served"], default="auto")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison run")
-    p.add_argument("--op", choices=["prop", "tomo", "shift"], default=None,
+    p.add_argument("--op", choices=["prop", "tomo", "shift", "bh"], default=None,
                    help="operator benchmark instead of the BH gradient: prop = D then DT, 16 fields of "
                         "256^2 at 4 distances; tomo = R and RT at 256^3, 384 angles; shift = coeff, S and "
-                        "Sadj at 384 x 256^2")
+                        "Sadj at 384 x 256^2; bh = Rec.BH iterations at 512^3, 720 angles, 1024^2")
     p.add_argument("--copy-calibration", action="store_true",
                    help="first run a 256 MiB device-to-device copy 3 times (a known-size kernel for "
                         "scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -348,6 +348,68 @@ def op_shift(a):
                     {"coeff": ms_c, "S": ms_s, "Sadj": ms_a})
 
 
+def op_bh(a):
+    """Rec.BH iterations on one GPU at 512^3 object, 720 angles, 1024^2 detector,
+    4 distances (16-angle launches, 16-slice tomography chunks, Laplacian
+    regulariser on).  One warm-up iteration (it has no beta step), then
+    `steps` timed iterations; the phases are timed with events on the launch
+    stream.  Bytes: the gradient's model bytes per angle times the angles, for
+    the roofline of the whole iteration."""
+    import numpy as np
+    import torch
+    from types import SimpleNamespace
+    from holotomocupy_amd import phantom
+    from holotomocupy_amd.rec import Rec
+    n, nobj, nth, nd = 1024, 512, 720, 4
+    g = geometry(n, nd)
+    theta = np.arange(nth) * np.pi / nth
+    args = SimpleNamespace(ndist=nd, ntheta=nth, nz=n, n=n, nzobj=nobj, nobj=nobj, obj_dtype="complex64",
+                           rho=[1.0, 0.05, 0.02], theta=theta, mask_r=0, niter=1 + a.steps, start_iter=0,
+                           err_step=-1, vis_step=-1, nchunk=16, lam_prbfit=0.0, lam_laplacian=1e-3, **g)
+    rec = Rec(args)
+    rng = np.random.default_rng(1003)
+    gen = torch.Generator(device="cuda").manual_seed(1003)
+    zz, yy, xx = [t.float() - nobj / 2 for t in torch.meshgrid(*(torch.arange(nobj, device="cuda"),) * 3,
+                                                                  indexing="ij")]
+    delta = torch.zeros((nobj, nobj, nobj), device="cuda")
+    for _ in range(30):
+        c = (torch.rand(3, generator=gen, device="cuda") - 0.5) * 0.7 * nobj
+        ax = (0.05 + 0.2 * torch.rand(3, generator=gen, device="cuda")) * nobj
+        inside = ((zz - c[0]) / ax[0]) ** 2 + ((yy - c[1]) / ax[1]) ** 2 + ((xx - c[2]) / ax[2]) ** 2 <= 1
+        delta += inside.float() * torch.rand(1, generator=gen, device="cuda")
+    del zz, yy, xx
+    obj = torch.complex(-delta, delta / 100)
+    del delta
+    proj = torch.empty((nth, nobj, nobj), dtype=torch.complex64, device="cuda")
+    rec.fwd_tomo(obj / float(rec.norm_const), out=proj)
+    obj *= 1.0 / float(proj.abs().max())  # phase of order one radian
+    del proj
+    prb = torch.from_numpy(phantom.probe(rng, nd, n, n)).cuda()
+    pos = torch.from_numpy(phantom.positions(rng, nth, nd)).cuda()
+    rec.gen_sqrt_data({"obj": obj, "prb": prb, "pos": pos})
+    rec.vars["obj"].copy_(obj * 0.8)
+    rec.vars["prb"].copy_(prb * 0.9)
+    rec.vars["pos"].copy_(pos + 0.3 * torch.randn(pos.shape, generator=gen, device="cuda"))
+    del obj
+    rec.time_phases = True
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rec.BH()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    timed = rec.phase_ms[1:]
+    parts = {k: sum(p[k] for p in timed) / len(timed) for k in timed[0]}
+    ms = sum(parts.values())
+    F, Fo = 8 * n * n, 8 * nobj * nobj
+    nbytes = nth * (nd * (6.5 * F + 2 * Fo) + 9 * Fo)
+    line = _op_line("BH iterations/s (object 512^3, 720 angles, 1024^2 detector, 4 distances)", 1e3 / ms,
+                    "iterations/s", ms, nbytes, "Rec.BH: gradient, R of the gradient, three bilinear forms, "
+                    "updates; object 512^3, 720 angles, detector 1024^2, 4 distances, Laplacian on", a, parts)
+    line["wall_s_all_iterations"] = round(wall, 3)
+    line["alphas"] = [float(x) for x in rec.alphas]
+    return line
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -357,7 +419,7 @@ def main():
     import torch
     torch.cuda.set_device(0)
     if a.op:
-        print(json.dumps({"prop": op_prop, "tomo": op_tomo, "shift": op_shift}[a.op](a)), flush=True)
+        print(json.dumps({"prop": op_prop, "tomo": op_tomo, "shift": op_shift, "bh": op_bh}[a.op](a)), flush=True)
         return
     rec, vars_, data, host = make_inputs(a)
     grads = {"prb": torch.zeros_like(vars_["prb"]), "proj": torch.empty_like(vars_["proj"]),

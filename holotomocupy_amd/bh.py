@@ -71,3 +71,33 @@ class BHGradient:
                   _lib.ptr(gproj), _lib.ptr(gpos), _lib.ptr(gprb),
                   _lib.ptr(objective) if objective is not None else None,
                   _lib.ptr(ws), ws.numel(), _lib.stream_ptr())
+
+    def hessian_workspace(self, nangles, same):
+        nb = ctypes.c_int64()
+        _lib.call("htamd_bh_hessian_workspace_size", self._plan, int(nangles), 1 if same else 0, ctypes.byref(nb))
+        if getattr(self, "_hws", None) is None or self._hws.numel() < nb.value:
+            self._hws = None  # release the old one first
+            self._hws = torch.empty(nb.value, dtype=torch.uint8, device=self.device)
+        return self._hws
+
+    def hessian(self, x, y, z, data, mag, out, accumulate=False):
+        """Fused bilinear form of the data term (Rec.hessian_cascade,
+        rec_mpi.py:250-303) for the angles of x['proj']: out (float64 device
+        tensor [1]) (+)= 2/data_size sum [...] (htamd_bh_hessian).  x, y, z are
+        {'prb', 'proj', 'pos'} dicts of device tensors; z is y means the same
+        direction (three propagations instead of four)."""
+        same = z is y
+        B = x["proj"].shape[0]
+        parts = [x["proj"], x["pos"], mag, x["prb"], y["proj"], y["pos"], y["prb"]]
+        if not same:
+            parts += [z["proj"], z["pos"], z["prb"]]
+        for t in parts + [data, out]:
+            if not t.is_contiguous() or t.device != self.device:
+                raise ValueError(f"bh hessian: every operand must be a contiguous tensor on {self.device}")
+        ws = self.hessian_workspace(B, same)
+        zz = [None, None, None] if same else [_lib.ptr(z["proj"]), _lib.ptr(z["pos"]), _lib.ptr(z["prb"])]
+        _lib.call("htamd_bh_hessian", self._plan, B, _lib.ptr(x["proj"]), _lib.ptr(x["pos"]), _lib.ptr(mag),
+                  _lib.ptr(x["prb"]), _lib.ptr(y["proj"]), _lib.ptr(y["pos"]), _lib.ptr(y["prb"]), *zz,
+                  _lib.ptr(data), float(2.0 / self.rec.data_size), _lib.ptr(out), 1 if accumulate else 0,
+                  _lib.ptr(ws), ws.numel(), _lib.stream_ptr())
+        return out

@@ -4,8 +4,9 @@ Drop-in for Rec.BH and what it calls (src/holotomocupy/rec_mpi.py:103-325,
 400-507, 815-902) on one GPU: the whole problem lives in HBM (object, the
 per-angle projections, the data, gradients and directions), so the reference's
 pinned host buffers, MPI redistribution and all-reduces fall away.  The
-gradient of each chunk of angles is the fused three-pass HIP pipeline; the
-bilinear form runs on the per-operator kernels; R / RT are the USFFT kernels;
+gradient of each chunk of angles is the fused three-pass HIP pipeline and the
+bilinear form the fused two-pass one (fused_hessian = False selects the
+per-operator cascade instead); R / RT are the USFFT kernels;
 the regulariser is the library's 3-D Laplacian.  The step sizes alpha and beta
 stay on the device: every inner product accumulates into float64 device
 scalars and the axpy updates read their coefficients there, so an iteration
@@ -176,10 +177,14 @@ class BHLoop:
     def hessian_dev(self, vars, grads, etas):
         """hessian (rec_mpi.py:235-248) as a float64 device scalar."""
         out = self._scalar()
+        fused = getattr(self, "fused_hessian", True)
         for st, B in self._chunks():
             g = self._sl(grads, st, B)
             e = g if grads is etas else self._sl(etas, st, B)
-            out += self._hessian_cascade_dev(self._sl(vars, st, B), g, e, self.data[st:st + B], st)
+            if fused:
+                self.hessian_fused(self._sl(vars, st, B), g, e, self.data[st:st + B], st, out=out)
+            else:
+                out += self._hessian_cascade_dev(self._sl(vars, st, B), g, e, self.data[st:st + B], st)
         self._hessian_prbfit_dev(vars["prb"], grads["prb"], etas["prb"], out)
         self.hessian_laplacian(grads["obj"], etas["obj"], out=out)
         return out
@@ -249,26 +254,54 @@ class BHLoop:
             self.error_debug(vars, -1)
             self.time_start = time.time()
             self.alphas, self.betas = [], []
+            timing = bool(getattr(self, "time_phases", False))
+            marks = []
+
+            def mark(name):
+                # phase boundary: an event on the launch stream (no host wait)
+                if timing:
+                    e = torch.cuda.Event(enable_timing=True)
+                    e.record()
+                    marks.append((name, e))
+
             for i in range(self.start_iter, self.niter):
+                mark("start")
                 self.gradients(vars, grads)
+                mark("gradients")
                 self.fwd_tomo(grads["obj"], out=grads["proj"])
+                mark("fwd_tomo")
                 if i == self.start_iter:
                     beta = self._scalar()
                 else:
                     beta = self.hessian_dev(vars, grads, etas) / self.hessian_dev(vars, etas, etas)
+                mark("hessian_beta")
                 ab = torch.cat([beta, self._scalar(-1.0)])
                 top = self._scalar()
                 for v in ["obj", "pos", "prb"]:
                     self._axpby_redot_dc(etas[v], grads[v], ab, top, -1.0 / self.rho_sq[v])
                 self._axpby_dc(etas["proj"], grads["proj"], ab, etas["proj"])
+                mark("direction")
                 alpha = top / self.hessian_dev(vars, etas, etas)
+                mark("hessian_alpha")
                 ab = torch.cat([self._scalar(1.0), alpha])
                 for v in ["obj", "prb", "pos", "proj"]:
                     self._axpby_dc(vars[v], etas[v], ab, vars[v])
+                mark("update")
                 self.alphas.append(alpha)
                 self.betas.append(beta)
                 self.error_debug(vars, i)
                 self.vis_debug(vars, i, writer)
+            if timing:
+                # per iteration, ms spent in each phase (from its previous mark)
+                torch.cuda.synchronize()
+                self.phase_ms = []
+                for name, e in marks:
+                    if name == "start":
+                        self.phase_ms.append({})
+                        prev = e
+                        continue
+                    self.phase_ms[-1][name] = prev.elapsed_time(e)
+                    prev = e
         finally:
             vars["obj"] *= float(self.norm_const)
         return vars

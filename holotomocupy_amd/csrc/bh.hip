@@ -15,10 +15,7 @@
 //               y22 and the shift-gradient partial sums            -> T
 //   K4 object:  gather-form Sadj summed over distances              -> dpsi
 // then the B-spline prefilter of dpsi and two small deterministic reductions.
-#include "../../include/htamd.h"
-#include "fresnel_core.hpp"
-#include "htamd_bspline.hpp"
-#include "htamd_internal.hpp"
+#include "bh_common.hpp"
 
 namespace htamd {
 
@@ -43,13 +40,6 @@ struct BhArgs {
   const float2* tabx = nullptr;      // [ndist][2][3*(n/2)]
   const float2* taby = nullptr;      // [ndist][2][3*(nz/2)]
 };
-
-template <typename TO> __device__ __forceinline__ float2 to_c(TO v);
-template <> __device__ __forceinline__ float2 to_c<float>(float v) { return make_float2(v, 0.f); }
-template <> __device__ __forceinline__ float2 to_c<float2>(float2 v) { return v; }
-template <typename TO> __device__ __forceinline__ TO from_c(float2 v);
-template <> __device__ __forceinline__ float from_c<float>(float2 v) { return v.x; }
-template <> __device__ __forceinline__ float2 from_c<float2>(float2 v) { return v; }
 
 // 16-tap B-spline gather at sampling point x of a row whose y-taps start at iy
 // with weights wy (and dwy).  v = sum wy wx c; with DERIV also
@@ -93,20 +83,6 @@ __device__ __forceinline__ void s_gather(const TO* __restrict__ c, int npsi, int
 // LDS values instead of 16 scattered global ones.  cy[x - lo] = sum_jy wy c,
 // cdy[x - lo] = sum_jy dwy c over the columns [lo, hi) the row's taps reach
 // (the same sums as s_gather, combined along y first).
-__device__ __forceinline__ unsigned char* dyn_lds() {
-  extern __shared__ __align__(16) unsigned char htamd_dyn_lds[];
-  return htamd_dyn_lds;
-}
-
-template <int N>
-__device__ __forceinline__ void row_span(int npsi, float mb, float rx, int& lo, int& hi) {
-  // bs_coord is monotone in the pixel index, so the row's taps lie within
-  // [floor(x(0)) - 1, floor(x(N-1)) + 2]
-  const float x0 = bs_coord(0, N, npsi, mb, rx), x1 = bs_coord(N - 1, N, npsi, mb, rx);
-  lo = max((int)floorf(fminf(x0, x1)) - 1, 0);
-  hi = min((int)floorf(fmaxf(x0, x1)) + 3, npsi);
-}
-
 template <typename TO, int T, bool DERIV>
 __device__ __forceinline__ void stage_row(const TO* __restrict__ cb, int npsi, int nzpsi, int iy,
                                           const float* wy, const float* dwy, int lo, int hi, int t,
@@ -510,7 +486,6 @@ __global__ __launch_bounds__(256) void bh_sum_k(const double* __restrict__ part,
 }
 
 // ---- host side --------------------------------------------------------------
-static int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
 
 struct BhLayout {
   int64_t c = 0, T = 0, dpsi = 0, part_r = 0, part_obj = 0, total = 0;
@@ -671,17 +646,6 @@ static hipError_t cols_dispatch(const BhArgs& a, hipStream_t st) {
   }
 }
 
-// Optional timing hook: events[2k] / events[2k+1] are recorded before / after
-// stage k (0 prefilter, 1 rows forward, 2 columns, 3 rows adjoint, 4 Sadj,
-// 5 prefilter of the gradient, 6 reductions) on the launch stream.
-struct StageEvents {
-  hipEvent_t* ev = nullptr;
-  int count = 0;
-  void mark(int idx, hipStream_t st) const {
-    if (ev && idx < count && ev[idx]) (void)hipEventRecord(ev[idx], st);
-  }
-};
-
 template <typename TO>
 static hipError_t bh_run(const htamd_shift_plan* sp, BhArgs a, const void* proj, void* gproj,
                          float* gpos, double* objective, const BhLayout& L, char* ws, hipStream_t st,
@@ -729,12 +693,6 @@ static hipError_t bh_run(const htamd_shift_plan* sp, BhArgs a, const void* proj,
 }  // namespace htamd
 
 using namespace htamd;
-
-struct htamd_bh_plan {
-  const htamd_prop_plan* prop = nullptr;
-  const htamd_shift_plan* shift = nullptr;
-  StageEvents events;
-};
 
 extern "C" {
 

@@ -1,0 +1,555 @@
+// Fused BH bilinear form of the holotomography data term on gfx950.
+//
+// Computes what Rec.hessian_cascade computes for a chunk of angles
+// (src/holotomocupy/rec_mpi.py:250-303, cascade 517-813): with the point
+// x = (prb, proj, pos) and directions y, z,
+//   c = coeff(proj), cy = coeff(proj_y), cz = coeff(proj_z)
+//   x22 = S(c),  y22 = dS(c; cy, dr_y),  z22 = dS(c; cz, dr_z),
+//   w22 = d2S(c; cy, dr_y, cz, dr_z)            (cuda_kernels.py:377-450, pairing kept)
+//   e = exp(i x22), y12 = i e y22, z12 = i e z22, w12 = e (i w22 - y22 z22)
+//   x0 = D(p e), y0 = D(py e + p y12), z0 = D(pz e + p z12),
+//   w0 = D(py z12 + pz y12 + p w12)
+//   H = 2/N sum [(1 - d/|x0|) Re(conj(y0) z0)
+//                + d/|x0|^3 Re(conj(x0) y0) Re(conj(x0) z0) + (1 - d/|x0|) Re(conj(x0) w0)]
+// (rec_mpi.py:539-569 for the last line).  With z = y the z fields are the y
+// fields and w0 = D(2 py y12 + p w12), as the reference's y_is_z branch.
+//
+// Passes, per (angle, distance):
+//   H1 rows:    staged-row gathers of c, cy, cz with the y weights and their
+//               derivatives, the four (three) input fields, D_x of each  -> fields
+//   H2 columns: D_y of each field and the pointwise sum; one FP64 partial
+//               per column tile, summed in a fixed order
+// The shift terms are sums over the same 16 taps as S: along y they are
+// combined once per (angle, row) into five LDS rows
+//   R0 = sum wy c,  Ay = sum wy cy - dry_y sum dwy c,  Az (same with z),
+//   W1 = cross sum dwy c - dry_x sum wy cy - drz_x sum wy cz,
+//   W0 = dry_y drz_y sum d2wy c - dry_y sum dwy cy - drz_y sum dwy cz
+// so that per pixel  x22 = sum wx R0,  y22 = sum wx Ay - dry_x sum dwx R0,
+//   w22 = dry_x drz_x sum d2wx R0 + sum dwx W1 + sum wx W0.
+#include <string>
+
+#include "bh_common.hpp"
+
+namespace htamd {
+
+struct BhhArgs {
+  int n = 0, nz = 0, npsi = 0, nzpsi = 0, ndist = 0;
+  int64_t nang = 0;
+  const void* c = nullptr;      // prefiltered projections, [nang, nzpsi, npsi] (object type)
+  const void* cy = nullptr;     // prefiltered direction y projections
+  const void* cz = nullptr;     // prefiltered direction z projections (== cy when z is y)
+  const float* pos = nullptr;   // [nang, ndist, 2] (y, x)
+  const float* dy = nullptr;    // direction y positions
+  const float* dz = nullptr;    // direction z positions
+  const float* mag = nullptr;   // [nang, ndist]
+  const float2* prb = nullptr;  // [ndist, nz, n]
+  const float2* py = nullptr;
+  const float2* pz = nullptr;
+  const float* data = nullptr;  // [nang, ndist, nz, n]
+  float2* F[4] = {nullptr, nullptr, nullptr, nullptr};  // x, y, z, w fields [nang, ndist, nz, n]
+  double* part = nullptr;       // [column tiles]
+  const float2* twx = nullptr;
+  const float2* twy = nullptr;
+  const float2* tabx = nullptr;
+  const float2* taby = nullptr;
+};
+
+__device__ __forceinline__ float2 mul_i(float2 v) { return make_float2(-v.y, v.x); }
+
+// ---- H1: rows ---------------------------------------------------------------
+// rpb row units per workgroup (rpb * N/E threads, rpb <= kRowThreads / (N/E)),
+// chosen so that the staged rows fit the dynamic LDS
+template <int N, int E, typename TO, bool SAME>
+__global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
+  constexpr int T = N / E, ROWS = kRowThreads / T;
+  constexpr int NR = SAME ? 4 : 5;  // staged rows per row unit: R0, Ay, (Az), W1, W0
+  constexpr int NF = SAME ? 3 : 4;  // fields: x, y, (z), w
+  __shared__ float2 lds[ROWS * fft_buf<N>()];
+  const int r = threadIdx.x / T, t0 = threadIdx.x % T;
+  const int64_t nunits = (int64_t)a.ndist * a.nz;
+  const int64_t unit = xcd_tile(blockIdx.x, gridDim.x) * rpb + r;
+  const bool valid = unit < nunits;
+  const int j = valid ? (int)(unit / a.nz) : 0, ty = valid ? (int)(unit % a.nz) : 0;
+  float2* s = lds + r * fft_buf<N>();
+  FftTw<N, E> twr;
+  fft_load_twiddles<N, E>(twr, t0, a.twx);
+  const float2* tab = a.tabx + (int64_t)(j * 2 + 0) * 3 * (N / 2);
+  const int64_t plane = (int64_t)a.nzpsi * a.npsi;
+  TO* rows = (TO*)dyn_lds() + (int64_t)r * NR * a.npsi;
+  TO* R0 = rows;
+  TO* Ay = rows + a.npsi;
+  TO* Az = SAME ? Ay : rows + 2 * a.npsi;
+  TO* W1 = rows + (NR - 2) * a.npsi;
+  TO* W0 = rows + (NR - 1) * a.npsi;
+  const int64_t prow = ((int64_t)j * a.nz + ty) * N;
+  for (int64_t b = 0; b < a.nang; ++b) {
+    int t = t0;
+    asm volatile("" : "+v"(t));
+    const int64_t bj = b * a.ndist + j;
+    const float mb = a.mag[bj], ry = a.pos[2 * bj], rx = a.pos[2 * bj + 1];
+    const float yyx = a.dy[2 * bj + 1], yyy = a.dy[2 * bj];
+    const float zzx = SAME ? yyx : a.dz[2 * bj + 1], zzy = SAME ? yyy : a.dz[2 * bj];
+    const float cross = yyx * zzy + yyy * zzx;
+    const float y = bs_coord(ty, a.nz, a.nzpsi, mb, ry);
+    const float fy = floorf(y);
+    const int iy = (int)fy - 1;
+    float wy[4], dwy[4], d2wy[4];
+    bs_w(y - fy, wy);
+    bs_dw(y - fy, dwy);
+    bs_d2w(y - fy, d2wy);
+    const TO* cb = (const TO*)a.c + b * plane;
+    const TO* cyb = (const TO*)a.cy + b * plane;
+    const TO* czb = (const TO*)a.cz + b * plane;
+    int lo, hi;
+    row_span<N>(a.npsi, mb, rx, lo, hi);
+    for (int x = lo + t; x < hi; x += T) {
+      TO r0 = vzero<TO>(), r1 = vzero<TO>(), r2 = vzero<TO>();
+      TO y0 = vzero<TO>(), y1 = vzero<TO>(), z0 = vzero<TO>(), z1 = vzero<TO>();
+#pragma unroll
+      for (int jy = 0; jy < 4; ++jy) {
+        const int yy = iy + jy;
+        if (yy < 0 || yy >= a.nzpsi) continue;
+        const int64_t k = (int64_t)yy * a.npsi + x;
+        const TO cv = cb[k], cyv = cyb[k];
+        r0 = vfma(wy[jy], cv, r0);
+        r1 = vfma(dwy[jy], cv, r1);
+        r2 = vfma(d2wy[jy], cv, r2);
+        y0 = vfma(wy[jy], cyv, y0);
+        y1 = vfma(dwy[jy], cyv, y1);
+        if constexpr (!SAME) {
+          const TO czv = czb[k];
+          z0 = vfma(wy[jy], czv, z0);
+          z1 = vfma(dwy[jy], czv, z1);
+        }
+      }
+      if constexpr (SAME) {
+        z0 = y0;
+        z1 = y1;
+      }
+      const int o = x - lo;
+      R0[o] = r0;
+      Ay[o] = vfma(-yyy, r1, y0);
+      if constexpr (!SAME) Az[o] = vfma(-zzy, r1, z0);
+      W1[o] = vfma(-zzx, z0, vfma(-yyx, y0, vfma(cross, r1, vzero<TO>())));
+      W0[o] = vfma(-zzy, z1, vfma(-yyy, y1, vfma(yyy * zzy, r2, vzero<TO>())));
+    }
+    __syncthreads();
+    // per-pixel values of the second-order chain, kept for the NF transforms
+    float2 ev[E], y12[E], z12[SAME ? 1 : E], w12[E];
+#pragma unroll 2
+    for (int e = 0; e < E; ++e) {
+      const int i = t + e * T;
+      const float x = bs_coord(i, N, a.npsi, mb, rx);
+      const float fx = floorf(x);
+      const int ix = (int)fx - 1;
+      float wx[4], dwx[4], d2wx[4];
+      bs_w(x - fx, wx);
+      bs_dw(x - fx, dwx);
+      bs_d2w(x - fx, d2wx);
+      TO x22 = vzero<TO>(), sx = vzero<TO>(), sxx = vzero<TO>(), ay = vzero<TO>(), az = vzero<TO>();
+      TO w1 = vzero<TO>(), w0 = vzero<TO>();
+#pragma unroll
+      for (int jx = 0; jx < 4; ++jx) {
+        const int xx = ix + jx;
+        if (xx < lo || xx >= hi) continue;  // within the row's span: xx outside [0, npsi)
+        const int o = xx - lo;
+        const TO rv = R0[o];
+        x22 = vfma(wx[jx], rv, x22);
+        sx = vfma(dwx[jx], rv, sx);
+        sxx = vfma(d2wx[jx], rv, sxx);
+        ay = vfma(wx[jx], Ay[o], ay);
+        if constexpr (!SAME) az = vfma(wx[jx], Az[o], az);
+        w1 = vfma(dwx[jx], W1[o], w1);
+        w0 = vfma(wx[jx], W0[o], w0);
+      }
+      const float2 yv = to_c(vfma(-yyx, sx, ay));
+      const float2 wv = to_c(vfma(yyx * zzx, sxx, vfma(1.0f, w1, w0)));
+      const float2 ex = expi_fast(to_c(x22));
+      ev[e] = ex;
+      y12[e] = cmul(ex, mul_i(yv));
+      if constexpr (SAME) {
+        const float2 yz = cmul(yv, yv);
+        w12[e] = cmul(ex, make_float2(-wv.y - yz.x, wv.x - yz.y));
+      } else {
+        const float2 zv = to_c(vfma(-zzx, sx, az));
+        z12[e] = cmul(ex, mul_i(zv));
+        const float2 yz = cmul(yv, zv);
+        w12[e] = cmul(ex, make_float2(-wv.y - yz.x, wv.x - yz.y));
+      }
+    }
+#pragma unroll 1
+    for (int k = 0; k < NF; ++k) {
+      const int kk = SAME && k == 2 ? 3 : k;  // field slot: x 0, y 1, z 2, w 3
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int i = t + e * T;
+        const float2 p = a.prb[prow + i];
+        float2 v;
+        if (kk == 0) {
+          v = cmul(p, ev[e]);
+        } else if (kk == 1) {
+          v = cadd(cmul(a.py[prow + i], ev[e]), cmul(p, y12[e]));
+        } else if (kk == 2) {
+          v = cadd(cmul(a.pz[prow + i], ev[e]), cmul(p, z12[SAME ? 0 : e]));
+        } else {
+          const float2 pyv = a.py[prow + i];
+          if constexpr (SAME)
+            v = cadd(cscale(cmul(pyv, y12[e]), 2.0f), cmul(p, w12[e]));
+          else
+            v = cadd(cadd(cmul(pyv, z12[e]), cmul(a.pz[prow + i], y12[e])), cmul(p, w12[e]));
+        }
+        s[fpad(dct_slot<N>(i))] = v;
+      }
+      __syncthreads();
+      fresnel_1d<N, E>(s, t, twr, tab);
+      if (valid) {
+        float2* dst = a.F[kk] + (bj * a.nz + ty) * (int64_t)N;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int i = t + e * T;
+          dst[i] = s[fpad(dct_slot<N>(i))];
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---- H2: columns ------------------------------------------------------------
+template <int N, int E, int C>
+__device__ __forceinline__ void hcol_transform(float2* lds, const float2* base, int n, int t, int cc,
+                                               const FftTw<N, E>& twr, const float2* tab) {
+  constexpr int T = N / E, NT = C * T, LS = col_stride<N, C>();
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int idx = threadIdx.x + e * NT;
+    const int c = idx % C, q = idx / C;  // FFT slot q holds row dct_nat(q)
+    lds[c * LS + fpad(q)] = base[dct_nat<N>(q) * n + c];
+  }
+  __syncthreads();
+  fresnel_1d<N, E>(lds + cc * LS, t, twr, tab);
+}
+
+template <int N, int E, int C, bool SAME>
+__global__ __launch_bounds__(C*(N / E)) void bhh_cols_k(BhhArgs a, int tiles_per_field) {
+  constexpr int T = N / E, NT = C * T, LS = col_stride<N, C>();
+  __shared__ float2 lds[C * LS];
+  __shared__ double red[NT / 64 > 0 ? NT / 64 : 1];
+  const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t f = tile / tiles_per_field;  // field b*ndist + j
+  const int x0c = (int)(tile % tiles_per_field) * C;
+  const int j = (int)(f % a.ndist);
+  const int cc = threadIdx.x / T, t = threadIdx.x % T;
+  FftTw<N, E> twr;
+  fft_load_twiddles<N, E>(twr, t, a.twy);
+  const int64_t off = f * (int64_t)N * a.n + x0c;
+  const float2* tab = a.taby + (int64_t)(j * 2 + 0) * 3 * (N / 2);
+  const float2* s = lds + cc * LS;
+  // x0 and sqrt(data) of this thread's elements
+  hcol_transform<N, E, C>(lds, a.F[0] + off, a.n, t, cc, twr, tab);
+  float2 xv[E];
+  float dv[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    xv[e] = s[fpad(t + e * T)];
+    dv[e] = a.data[off + (int64_t)dct_nat<N>(t + e * T) * a.n + cc];
+  }
+  __syncthreads();
+  // (1 - d/|x0|) Re(conj(x0) w0)
+  double acc = 0.0;
+  hcol_transform<N, E, C>(lds, a.F[3] + off, a.n, t, cc, twr, tab);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const float2 w = s[fpad(t + e * T)];
+    const float am = sqrtf(xv[e].x * xv[e].x + xv[e].y * xv[e].y);
+    acc += (double)((1.0f - dv[e] / am) * (xv[e].x * w.x + xv[e].y * w.y));
+  }
+  __syncthreads();
+  hcol_transform<N, E, C>(lds, a.F[1] + off, a.n, t, cc, twr, tab);
+  if constexpr (SAME) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const float2 yv = s[fpad(t + e * T)];
+      const float am = sqrtf(xv[e].x * xv[e].x + xv[e].y * xv[e].y);
+      const float xy = xv[e].x * yv.x + xv[e].y * yv.y;
+      const float t1 = (1.0f - dv[e] / am) * (yv.x * yv.x + yv.y * yv.y);
+      const float t2 = dv[e] / (am * am * am) * xy * xy;
+      acc += (double)(t1 + t2);
+    }
+  } else {
+    float2 yv[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) yv[e] = s[fpad(t + e * T)];
+    __syncthreads();
+    hcol_transform<N, E, C>(lds, a.F[2] + off, a.n, t, cc, twr, tab);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const float2 zv = s[fpad(t + e * T)];
+      const float am = sqrtf(xv[e].x * xv[e].x + xv[e].y * xv[e].y);
+      const float xy = xv[e].x * yv[e].x + xv[e].y * yv[e].y;
+      const float xz = xv[e].x * zv.x + xv[e].y * zv.y;
+      const float t1 = (1.0f - dv[e] / am) * (yv[e].x * zv.x + yv[e].y * zv.y);
+      const float t2 = dv[e] / (am * am * am) * xy * xz;
+      acc += (double)(t1 + t2);
+    }
+  }
+  if constexpr (NT >= 64) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double tot = 0.0;
+#pragma unroll
+      for (int k = 0; k < NT / 64; ++k) tot += red[k];
+      a.part[tile] = tot;
+    }
+  } else {
+    __shared__ double all[NT];
+    all[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double tot = 0.0;
+      for (int k = 0; k < NT; ++k) tot += all[k];
+      a.part[tile] = tot;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void bhh_sum_k(const double* __restrict__ part, int64_t count, double scale,
+                                                 int accumulate, double* __restrict__ out) {
+  __shared__ double r[256];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < count; i += 256) s += part[i];
+  r[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) r[threadIdx.x] += r[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = accumulate ? out[0] + r[0] * scale : r[0] * scale;
+}
+
+// ---- host side --------------------------------------------------------------
+// Column pass: 8 elements per thread (x0, sqrt(data) and y0 stay in registers
+// across the transforms), C columns per workgroup within 256 threads.
+template <int N>
+constexpr int bhh_col_e() {
+  return N >= 8 ? 8 : N;
+}
+
+template <int N>
+constexpr int bhh_cols_c() {
+  constexpr int T = N / bhh_col_e<N>();
+  return T >= 256 ? 1 : 256 / T > 4 ? 4 : 256 / T;
+}
+
+static int hess_cols_c(int64_t nz) {
+  switch (nz) {
+    case 8: return bhh_cols_c<8>();
+    case 16: return bhh_cols_c<16>();
+    case 32: return bhh_cols_c<32>();
+    case 64: return bhh_cols_c<64>();
+    case 128: return bhh_cols_c<128>();
+    case 256: return bhh_cols_c<256>();
+    case 512: return bhh_cols_c<512>();
+    case 1024: return bhh_cols_c<1024>();
+    case 2048: return bhh_cols_c<2048>();
+    default: return 1;
+  }
+}
+
+struct BhhLayout {
+  int64_t c = 0, cy = 0, cz = 0, F[4] = {0, 0, 0, 0}, part = 0, total = 0;
+};
+
+static BhhLayout bhh_layout(const htamd_prop_plan* pp, const htamd_shift_plan* sp, int64_t nang, bool same) {
+  BhhLayout L;
+  const int64_t osz = sp->is_complex ? 8 : 4;
+  const int64_t cbytes = align256(nang * sp->nzpsi * sp->npsi * osz);
+  const int64_t fbytes = align256(nang * pp->ndist * pp->n * pp->nz * 8);
+  int64_t off = 0;
+  L.c = off;
+  off += cbytes;
+  L.cy = off;
+  off += cbytes;
+  L.cz = same ? L.cy : off;
+  if (!same) off += cbytes;
+  for (int k = 0; k < 4; ++k) {
+    if (same && k == 2) {
+      L.F[2] = L.F[1];
+      continue;
+    }
+    L.F[k] = off;
+    off += fbytes;
+  }
+  const int C = hess_cols_c(pp->nz);
+  const int64_t ntiles = nang * pp->ndist * (pp->n / C);
+  L.part = off;
+  off += align256(ntiles * 8);
+  L.total = off;
+  return L;
+}
+
+// Row pass: 8 elements per thread as the gradient's row passes; the staged
+// rows of a workgroup stay within 64 KiB of dynamic LDS.
+constexpr int64_t kHessDynBudget = 65536;
+
+template <int N, typename TO>
+static hipError_t hess_rows(bool same, const BhhArgs& a, hipStream_t st) {
+  constexpr int E = N >= 8 ? 8 : N;
+  constexpr int T = N / E, ROWS = kRowThreads / T;
+  const int64_t per_row = (int64_t)(same ? 4 : 5) * a.npsi * (int64_t)sizeof(TO);
+  int rpb = ROWS;
+  while (rpb > 1 && rpb * per_row > kHessDynBudget) rpb >>= 1;
+  if (rpb * per_row > kHessDynBudget) return hipErrorInvalidConfiguration;
+  const int64_t grid = ((int64_t)a.ndist * a.nz + rpb - 1) / rpb;
+  const size_t dyn = (size_t)(rpb * per_row);
+  if (same)
+    hipLaunchKernelGGL((bhh_rows_k<N, E, TO, true>), dim3((unsigned)grid), dim3(rpb * T), dyn, st, a, rpb);
+  else
+    hipLaunchKernelGGL((bhh_rows_k<N, E, TO, false>), dim3((unsigned)grid), dim3(rpb * T), dyn, st, a, rpb);
+  return hipGetLastError();
+}
+
+template <typename TO>
+static hipError_t hess_rows_dispatch(bool same, const BhhArgs& a, hipStream_t st) {
+  switch (a.n) {
+    case 8: return hess_rows<8, TO>(same, a, st);
+    case 16: return hess_rows<16, TO>(same, a, st);
+    case 32: return hess_rows<32, TO>(same, a, st);
+    case 64: return hess_rows<64, TO>(same, a, st);
+    case 128: return hess_rows<128, TO>(same, a, st);
+    case 256: return hess_rows<256, TO>(same, a, st);
+    case 512: return hess_rows<512, TO>(same, a, st);
+    case 1024: return hess_rows<1024, TO>(same, a, st);
+    case 2048: return hess_rows<2048, TO>(same, a, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int NZ>
+static hipError_t hess_cols(bool same, const BhhArgs& a, hipStream_t st) {
+  constexpr int E = bhh_col_e<NZ>(), C = bhh_cols_c<NZ>();
+  if (a.n % C) return hipErrorInvalidValue;
+  const int tpf = a.n / C;
+  const int64_t grid = a.nang * a.ndist * tpf;
+  if (same)
+    hipLaunchKernelGGL((bhh_cols_k<NZ, E, C, true>), dim3((unsigned)grid), dim3(C * (NZ / E)), 0, st, a, tpf);
+  else
+    hipLaunchKernelGGL((bhh_cols_k<NZ, E, C, false>), dim3((unsigned)grid), dim3(C * (NZ / E)), 0, st, a, tpf);
+  return hipGetLastError();
+}
+
+static hipError_t hess_cols_dispatch(bool same, const BhhArgs& a, hipStream_t st) {
+  switch (a.nz) {
+    case 8: return hess_cols<8>(same, a, st);
+    case 16: return hess_cols<16>(same, a, st);
+    case 32: return hess_cols<32>(same, a, st);
+    case 64: return hess_cols<64>(same, a, st);
+    case 128: return hess_cols<128>(same, a, st);
+    case 256: return hess_cols<256>(same, a, st);
+    case 512: return hess_cols<512>(same, a, st);
+    case 1024: return hess_cols<1024>(same, a, st);
+    case 2048: return hess_cols<2048>(same, a, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename TO>
+static hipError_t bhh_run(const htamd_shift_plan* sp, bool same, BhhArgs a, const void* proj, const void* projy,
+                          const void* projz, char* ws, const BhhLayout& L, double scale, int accumulate,
+                          double* result, hipStream_t st) {
+  hipError_t e;
+  if ((e = bs_launch_prefilter<TO>((const TO*)proj, (TO*)(ws + L.c), a.nang, a.nzpsi, a.npsi, sp->taps, st)) !=
+      hipSuccess)
+    return e;
+  if ((e = bs_launch_prefilter<TO>((const TO*)projy, (TO*)(ws + L.cy), a.nang, a.nzpsi, a.npsi, sp->taps, st)) !=
+      hipSuccess)
+    return e;
+  if (!same &&
+      (e = bs_launch_prefilter<TO>((const TO*)projz, (TO*)(ws + L.cz), a.nang, a.nzpsi, a.npsi, sp->taps, st)) !=
+          hipSuccess)
+    return e;
+  if ((e = hess_rows_dispatch<TO>(same, a, st)) != hipSuccess) return e;
+  if ((e = hess_cols_dispatch(same, a, st)) != hipSuccess) return e;
+  const int64_t ntiles = a.nang * a.ndist * (a.n / hess_cols_c(a.nz));
+  hipLaunchKernelGGL(bhh_sum_k, dim3(1), dim3(256), 0, st, a.part, ntiles, scale, accumulate, result);
+  return hipGetLastError();
+}
+
+}  // namespace htamd
+
+using namespace htamd;
+
+extern "C" {
+
+int htamd_bh_hessian_workspace_size(const htamd_bh_plan* p, int64_t nangles, int same_directions,
+                                    int64_t* bytes) {
+  if (!p || !bytes || nangles < 0) return fail(HTAMD_ERR_ARG, "bh_hessian_workspace_size: bad arguments");
+  *bytes = bhh_layout(p->prop, p->shift, nangles, same_directions != 0).total;
+  return HTAMD_OK;
+}
+
+int htamd_bh_hessian(const htamd_bh_plan* p, int64_t nangles, const void* proj, const float* pos, const float* mag,
+                     const void* prb, const void* proj_y, const float* pos_y, const void* prb_y, const void* proj_z,
+                     const float* pos_z, const void* prb_z, const float* data, double scale, double* result,
+                     int accumulate, void* workspace, int64_t workspace_bytes, void* stream) {
+  if (!p) return fail(HTAMD_ERR_ARG, "bh_hessian: plan is NULL");
+  if (nangles < 0 || !result) return fail(HTAMD_ERR_ARG, "bh_hessian: bad arguments");
+  const bool same = proj_z == nullptr && pos_z == nullptr && prb_z == nullptr;
+  if (nangles == 0) {
+    if (!accumulate) {
+      hipLaunchKernelGGL(bhh_sum_k, dim3(1), dim3(256), 0, (hipStream_t)stream, (const double*)nullptr, 0, 0.0,
+                         0, result);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return hip_fail(e, "bh_hessian");
+    }
+    return HTAMD_OK;
+  }
+  if (!proj || !pos || !mag || !prb || !proj_y || !pos_y || !prb_y || !data)
+    return fail(HTAMD_ERR_ARG, "bh_hessian: NULL buffer");
+  if (!same && (!proj_z || !pos_z || !prb_z))
+    return fail(HTAMD_ERR_ARG, "bh_hessian: give all three parts of direction z, or none (z = y)");
+  const BhhLayout L = bhh_layout(p->prop, p->shift, nangles, same);
+  if (!workspace || workspace_bytes < L.total)
+    return fail(HTAMD_ERR_ARG, "bh_hessian: workspace of " + std::to_string(L.total) + " bytes required");
+  int dev = -1;
+  (void)hipGetDevice(&dev);
+  if (dev != p->prop->device) return fail(HTAMD_ERR_ARG, "bh_hessian: plan belongs to another device");
+  char* ws = (char*)workspace;
+  BhhArgs a;
+  a.n = (int)p->prop->n;
+  a.nz = (int)p->prop->nz;
+  a.npsi = (int)p->shift->npsi;
+  a.nzpsi = (int)p->shift->nzpsi;
+  a.ndist = (int)p->prop->ndist;
+  a.nang = nangles;
+  a.c = ws + L.c;
+  a.cy = ws + L.cy;
+  a.cz = ws + L.cz;
+  a.pos = pos;
+  a.dy = pos_y;
+  a.dz = same ? pos_y : pos_z;
+  a.mag = mag;
+  a.prb = (const float2*)prb;
+  a.py = (const float2*)prb_y;
+  a.pz = (const float2*)(same ? prb_y : prb_z);
+  a.data = data;
+  for (int k = 0; k < 4; ++k) a.F[k] = (float2*)(ws + L.F[k]);
+  a.part = (double*)(ws + L.part);
+  a.twx = p->prop->tw_x;
+  a.twy = p->prop->tw_y;
+  a.tabx = p->prop->tab_x;
+  a.taby = p->prop->tab_y;
+  hipError_t e = p->shift->is_complex
+      ? bhh_run<float2>(p->shift, same, a, proj, proj_y, proj_z, ws, L, scale, accumulate, result,
+                        (hipStream_t)stream)
+      : bhh_run<float>(p->shift, same, a, proj, proj_y, proj_z, ws, L, scale, accumulate, result,
+                       (hipStream_t)stream);
+  if (e == hipErrorInvalidConfiguration)
+    return fail(HTAMD_ERR_UNSUPPORTED, "bh_hessian: the staged object rows of one row pass exceed the LDS");
+  return e == hipSuccess ? HTAMD_OK : hip_fail(e, "bh_hessian");
+}
+
+}  // extern "C"

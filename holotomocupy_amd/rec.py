@@ -316,6 +316,21 @@ class Rec(BHLoop):
         self._bh(vars["prb"], vars["proj"], vars["pos"], data, mag, grads["proj"], grads["pos"],
                  grads["prb"], accumulate_prb=True, objective=objective)
 
+    def hessian_fused(self, vars, grads, etas, data=None, st=0, out=None):
+        """hessian_cascade through the fused two-pass HIP pipeline
+        (htamd_bh_hessian), as a float64 device tensor [1] (added into out if
+        given).  grads is etas selects the equal-direction form."""
+        from .bh import BHGradient
+        data = self.data if data is None else data
+        B = vars["proj"].shape[0]
+        if getattr(self, "_bh", None) is None:
+            self._bh = BHGradient(self)
+        mag = self.eff_demagnifications[st:st + B].contiguous()
+        res = torch.zeros(1, dtype=torch.float64, device=self.device) if out is None else out
+        z = grads if grads is etas else etas
+        self._bh.hessian(vars, grads, z, data, mag, res, accumulate=True)
+        return res
+
     def hessian_cascade(self, vars, grads, etas, data=None, st=0):
         """BH bilinear form of the data term (rec_mpi.py:250-303), as a float."""
         return float(self._hessian_cascade_dev(vars, grads, etas, data, st))

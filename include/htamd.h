@@ -223,6 +223,24 @@ int htamd_bh_gradient(const htamd_bh_plan* plan, int64_t nangles, const void* pr
                       float* gpos, void* gprb, double* objective, void* workspace,
                       int64_t workspace_bytes, void* stream);
 
+/* Fused BH bilinear form of the data term.  Replaces Rec.hessian_cascade / _hessian_cascade and the
+ * dF1..dF3, d2F_dF0..d2F_dF3 chain (src/holotomocupy/rec_mpi.py:250-303, 517-813) for one chunk of
+ * angles: result (+)= scale * sum [(1 - d/|x0|) Re(conj(y0) z0) + d/|x0|^3 Re(conj(x0) y0) Re(conj(x0) z0)
+ * + (1 - d/|x0|) Re(conj(x0) w0)], with x0, y0, z0, w0 the propagated fields of the point and of the
+ * first- and second-order perturbations along directions y = (prb_y, proj_y, pos_y) and
+ * z = (prb_z, proj_z, pos_z); scale = 2/data_size.  proj_z, pos_z, prb_z all NULL means z = y (the
+ * reference's y_is_z branch: three propagations instead of four).  The second-order shift term keeps
+ * the reference's d2s pairing (cuda_kernels.py:377-450).  Layouts as htamd_bh_gradient (one probe per
+ * distance); result is one device double.  Deterministic (fixed-order FP64 partial sums).  The
+ * workspace holds the prefiltered projections and the propagated fields of the chunk. */
+int htamd_bh_hessian_workspace_size(const htamd_bh_plan* plan, int64_t nangles, int same_directions,
+                                    int64_t* bytes);
+int htamd_bh_hessian(const htamd_bh_plan* plan, int64_t nangles, const void* proj, const float* pos,
+                     const float* mag, const void* prb, const void* proj_y, const float* pos_y,
+                     const void* prb_y, const void* proj_z, const float* pos_z, const void* prb_z,
+                     const float* data, double scale, double* result, int accumulate, void* workspace,
+                     int64_t workspace_bytes, void* stream);
+
 /* ---- BLAS-1 helpers of the BH iteration ----------------------------------------
  * Arrays are float streams of n floats (a complex64 array of m elements is
  * n = 2m); 16-byte aligned pointers take a vectorised path.  Reductions are deterministic
