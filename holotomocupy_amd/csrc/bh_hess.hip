@@ -136,8 +136,7 @@ __global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
     __syncthreads();
     // per-pixel values of the second-order chain, kept for the NF transforms
     float2 ev[E], y12[E], z12[SAME ? 1 : E], w12[E];
-#pragma unroll 2
-    for (int e = 0; e < E; ++e) {
+    auto chain = [&](int e) {
       const int i = t + e * T;
       const float x = bs_coord(i, N, a.npsi, mb, rx);
       const float fx = floorf(x);
@@ -176,10 +175,20 @@ __global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
         const float2 yz = cmul(yv, zv);
         w12[e] = cmul(ex, make_float2(-wv.y - yz.x, wv.x - yz.y));
       }
+    };
+    // equal directions: full unroll keeps the three arrays in registers; the
+    // general form's four arrays measure faster in scratch with a compact loop
+    if constexpr (SAME) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) chain(e);
+    } else {
+#pragma unroll 2
+      for (int e = 0; e < E; ++e) chain(e);
     }
+    // fields in the order w, y, z, x, one LDS transform each
 #pragma unroll 1
     for (int k = 0; k < NF; ++k) {
-      const int kk = SAME && k == 2 ? 3 : k;  // field slot: x 0, y 1, z 2, w 3
+      const int kk = k == 0 ? 3 : k == NF - 1 ? 0 : k;  // field slot: x 0, y 1, z 2, w 3
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int i = t + e * T;
