@@ -12,8 +12,9 @@ arrays (the reference's "direct" mode, chunking.py:40-42).
 Pipeline: three HIP streams (default priority) carry host-to-device copies of
 chunk k, the computation on chunk k-1 and device-to-host copies of chunk k-2
 at once, through page-locked staging buffers (double-buffered, reused across
-calls) and device buffers carved from one pool of ``nbytes``.  Streams are
-synchronised once per step, as in the reference.
+calls) and device buffers carved from one pool of ``nbytes``.  The stages are
+ordered by events (a stage waits only for the event that frees the buffer it
+overwrites), where the reference synchronises all three streams once per step.
 
 The BLAS-1 helpers (redot_batch, linear_batch, linear_redot_batch,
 mulc_batch) run the library's deterministic kernels (htamd_redot,
@@ -190,26 +191,52 @@ class Chunking:
                     sv.copy_(src, non_blocking=True)
                     pend[(buf, j)] = (dst, sv)
 
+        # Event-ordered pipeline: stream 0 copies chunks in, stream 1 computes,
+        # stream 2 copies results out.  Buffer j = k % 2 of chunk k is reused by
+        # chunk k + 2, so each stage waits only for the events that free what
+        # it overwrites; the host blocks only before refilling a page-locked
+        # input buffer (its copy of chunk k - 2 must have left) and before
+        # draining an output buffer (its chunk must have arrived).
+        h2d, comp, d2h = self.stream
         cur = torch.cuda.current_stream(self.device)
-        for s in self.stream:
-            s.wait_stream(cur)  # inputs produced on the caller's stream
-        for k in range(nchunk + 2):
-            if k < nchunk:
-                with torch.cuda.stream(self.stream[k % 3]):
-                    p2g(k % 2, k)
-            if 0 < k < nchunk + 1:
-                with torch.cuda.stream(self.stream[(k - 1) % 3]):
-                    compute((k - 1) % 2, k - 1)
-            if 1 < k:
-                with torch.cuda.stream(self.stream[(k - 2) % 3]):
-                    g2p((k - 2) % 2, k - 2)
-            for s in self.stream:
-                s.synchronize()
+        for s_ in self.stream:
+            s_.wait_stream(cur)  # inputs produced on the caller's stream
+        ev_in = [torch.cuda.Event() for _ in range(nchunk)]
+        ev_comp = [torch.cuda.Event() for _ in range(nchunk)]
+        ev_out = [torch.cuda.Event() for _ in range(nchunk)]
+
+        def drain(k):
+            ev_out[k].synchronize()
             for (buf, j), (dst, sv) in list(pend.items()):
-                out[j][dst] = sv.numpy()
-                del pend[(buf, j)]
-        for s in self.stream:
-            cur.wait_stream(s)
+                if buf == k % 2:
+                    out[j][dst] = sv.numpy()
+                    del pend[(buf, j)]
+
+        for k in range(nchunk + 3):
+            if k < nchunk:
+                if k >= 2:
+                    ev_in[k - 2].synchronize()  # page-locked input buffer k % 2 is free
+                with torch.cuda.stream(h2d):
+                    if k >= 2:
+                        h2d.wait_event(ev_comp[k - 2])  # device input buffer k % 2 is free
+                    p2g(k % 2, k)
+                    ev_in[k].record(h2d)
+            if 0 <= k - 1 < nchunk:
+                with torch.cuda.stream(comp):
+                    comp.wait_event(ev_in[k - 1])
+                    if k - 1 >= 2:
+                        comp.wait_event(ev_out[k - 3])  # device output buffer is free
+                    compute((k - 1) % 2, k - 1)
+                    ev_comp[k - 1].record(comp)
+            if 0 <= k - 2 < nchunk:
+                with torch.cuda.stream(d2h):
+                    d2h.wait_event(ev_comp[k - 2])
+                    g2p((k - 2) % 2, k - 2)
+                    ev_out[k - 2].record(d2h)
+            if 0 <= k - 3 < nchunk:
+                drain(k - 3)  # also frees the page-locked output buffer for chunk k - 1
+        for s_ in self.stream:
+            cur.wait_stream(s_)
 
     # ------------------------------------------------------------------ BLAS-1
     def _dev_redot(self, x, y, accumulate=False):

@@ -177,3 +177,24 @@ def test_bh_gradient_and_hessian_pieces(gpu):
           for k, v in e.items()}
     assert float(rec.hessian_dev(x, ed, ed)) == pytest.approx(orc.hessian(xo, e, e), rel=1e-4)
     assert rec.min(x["prb"], x["obj"], x["pos"], x["proj"]) == pytest.approx(orc.min(xo), rel=1e-5)
+
+
+def test_bh_ten_iterations_precision(gpu):
+    """Precision study of the loop: ten FP32 iterations (fused gradient and
+    bilinear forms, FP64 tables and reductions) against the FP64 restatement.
+    The objective after every iteration is compared at 1e-4 relative, and the
+    variables after ten iterations at 1e-3 relative L2 (small step-size
+    differences accumulate along the iterations)."""
+    print("[case] BH loop ten iterations precision", flush=True)
+    rec, orc, start = make_bh(False, 0.3, 0.05, niter=10, seed=23)
+    rec.BH()
+    torch.cuda.synchronize()
+    vo, hist = orc.BH(start, rec.niter)
+    errs = [e for i, e, _ in rec.table if i >= 0]
+    rels = [abs(a - h[2]) / abs(h[2]) for a, h in zip(errs, hist)]
+    print("objective rel. errors", ["%.2e" % r for r in rels], flush=True)
+    assert max(rels) < 1e-4
+    for k in ("obj", "prb", "pos"):
+        e = rel(rec.vars[k], vo[k])
+        print(k, e, flush=True)
+        assert e < 1e-3
