@@ -194,7 +194,7 @@ __global__ __launch_bounds__(kRowThreads) void bh_fwd_rows_k(BhArgs a) {
         const int i = t + e * T;
         TO v, d1, d2;
         row_gather<TO, false>(cy, cy, lo, hi, bs_coord(i, N, a.npsi, mb, rx), v, d1, d2);
-        s[fpad(dct_slot<N>(i))] = cmul(prow[i], expi_fast(to_c(v)));
+        s[row_slot<N, E>(t, e)] = cmul(prow[i], expi_fast(to_c(v)));
       }
     } else {
 #pragma unroll 1
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(kRowThreads) void bh_fwd_rows_k(BhArgs a) {
         const int i = t + e * T;
         TO v, d1, d2;
         s_gather<TO, false>(cb, a.npsi, a.nzpsi, bs_coord(i, N, a.npsi, mb, rx), iy, wy, wy, v, d1, d2);
-        s[fpad(dct_slot<N>(i))] = cmul(prow[i], expi_fast(to_c(v)));
+        s[row_slot<N, E>(t, e)] = cmul(prow[i], expi_fast(to_c(v)));
       }
     }
     __syncthreads();
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(kRowThreads) void bh_fwd_rows_k(BhArgs a) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int i = t + e * T;
-        dst[i] = s[fpad(dct_slot<N>(i))];
+        dst[i] = s[row_slot<N, E>(t, e)];
       }
     }
     __syncthreads();
@@ -235,12 +235,7 @@ __global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_f
   fft_load_twiddles<N, E>(twr, t, a.twy);
   float2* base = a.T + f * (int64_t)N * a.n + x0;
   const float* dbase = a.data + f * (int64_t)N * a.n + x0;
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int idx = threadIdx.x + e * NT;
-    const int c = idx % C, q = idx / C;  // FFT slot q holds row dct_nat(q)
-    lds[c * LS + fpad(q)] = base[dct_nat<N>(q) * a.n + c];
-  }
+  col_tile_xfer<N, E, C, true>(lds, base, a.n, threadIdx.x);  // FFT slot q holds row dct_nat(q)
   __syncthreads();
   float2* s = lds + cc * LS;
   fresnel_1d<N, E>(s, t, twr, a.taby + (int64_t)(j * 2 + 0) * 3 * (N / 2));
@@ -267,12 +262,7 @@ __global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_f
   fresnel_1d<N, E>(s, t, twr, a.taby + (int64_t)(j * 2 + 1) * 3 * (N / 2));
   int tid = threadIdx.x;
   asm volatile("" : "+v"(tid));
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int idx = tid + e * NT;
-    const int c = idx % C, q = idx / C;
-    base[dct_nat<N>(q) * a.n + c] = lds[c * LS + fpad(q)];
-  }
+  col_tile_xfer<N, E, C, false>(lds, base, a.n, tid);
   if constexpr (NT >= 64) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) obj += __shfl_xor(obj, off, 64);
@@ -337,7 +327,7 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = t + e * T;
-      s[fpad(dct_slot<N>(i))] = valid ? row[i] : make_float2(0.f, 0.f);
+      s[row_slot<N, E>(t, e)] = valid ? row[i] : make_float2(0.f, 0.f);
     }
     const float mb = a.mag[bj], ry = a.pos[2 * bj], rx = a.pos[2 * bj + 1];
     const float y = bs_coord(ty, a.nz, a.nzpsi, mb, ry);
@@ -366,7 +356,7 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
 #pragma unroll 1
     for (int e = 0; e < E; ++e) {
       const int i = t + e * T;
-      const float2 y12 = s[fpad(dct_slot<N>(i))];
+      const float2 y12 = s[row_slot<N, E>(t, e)];
       TO v, d1, d2;
       if constexpr (STAGED)
         row_gather<TO, true>(cy, cdy, lo, hi, bs_coord(i, N, a.npsi, mb, rx), v, d1, d2);

@@ -207,7 +207,7 @@ __global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
           else
             v = cadd(cadd(cmul(pyv, z12[e]), cmul(a.pz[prow + i], y12[e])), cmul(p, w12[e]));
         }
-        s[fpad(dct_slot<N>(i))] = v;
+        s[row_slot<N, E>(t, e)] = v;
       }
       __syncthreads();
       fresnel_1d<N, E>(s, t, twr, tab);
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           const int i = t + e * T;
-          dst[i] = s[fpad(dct_slot<N>(i))];
+          dst[i] = s[row_slot<N, E>(t, e)];
         }
       }
       __syncthreads();
@@ -228,13 +228,8 @@ __global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
 template <int N, int E, int C>
 __device__ __forceinline__ void hcol_transform(float2* lds, const float2* base, int n, int t, int cc,
                                                const FftTw<N, E>& twr, const float2* tab) {
-  constexpr int T = N / E, NT = C * T, LS = col_stride<N, C>();
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int idx = threadIdx.x + e * NT;
-    const int c = idx % C, q = idx / C;  // FFT slot q holds row dct_nat(q)
-    lds[c * LS + fpad(q)] = base[dct_nat<N>(q) * n + c];
-  }
+  constexpr int LS = col_stride<N, C>();
+  col_tile_xfer<N, E, C, true>(lds, const_cast<float2*>(base), n, threadIdx.x);  // slot q: row dct_nat(q)
   __syncthreads();
   fresnel_1d<N, E>(lds + cc * LS, t, twr, tab);
 }

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(kRowThreads) void fresnel_rows_k(const float2* in, 
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = t + e * T;
-    s[fpad(dct_slot<N>(i))] = valid ? src[i] : make_float2(0.f, 0.f);
+    s[row_slot<N, E>(t, e)] = valid ? src[i] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   fresnel_1d<N, E>(s, t, twr, tab);
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kRowThreads) void fresnel_rows_k(const float2* in, 
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = t + e * T;
-      dst[i] = s[fpad(dct_slot<N>(i))];
+      dst[i] = s[row_slot<N, E>(t, e)];
     }
   }
 }
@@ -74,20 +74,10 @@ __global__ __launch_bounds__(C*(N / E)) void fresnel_cols_k(float2* data, int64_
   const int cc = threadIdx.x / T, t = threadIdx.x % T;
   FftTw<N, E> twr;
   fft_load_twiddles<N, E>(twr, t, tw);
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int idx = threadIdx.x + e * NT;
-    const int c = idx % C, q = idx / C;  // FFT slot q holds row dct_nat(q)
-    lds[c * LS + fpad(q)] = base[(int64_t)dct_nat<N>(q) * n + c];
-  }
+  col_tile_xfer<N, E, C, true>(lds, base, n, threadIdx.x);  // FFT slot q holds row dct_nat(q)
   __syncthreads();
   fresnel_1d<N, E>(lds + cc * LS, t, twr, tab);
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int idx = threadIdx.x + e * NT;
-    const int c = idx % C, q = idx / C;
-    base[(int64_t)dct_nat<N>(q) * n + c] = lds[c * LS + fpad(q)];
-  }
+  col_tile_xfer<N, E, C, false>(lds, base, n, threadIdx.x);
 }
 
 bool fft_size_supported(int64_t n) {

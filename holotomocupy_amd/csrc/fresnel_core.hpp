@@ -54,4 +54,61 @@ constexpr int col_e() {
 
 constexpr int kRowThreads = 256;
 
+// Global rows of a column transform held in Makhoul order.  FFT slot
+// q = q0 + e*T (T = N/E, 0 <= q0 < T, e = 0..E-1) holds natural row dct_nat(q),
+// which is 2q for e < E/2 and 2N-1-2q for e >= E/2: two affine runs in e, so
+// the element offsets row*n + c need one multiply-add each instead of the
+// index arithmetic of dct_nat.
+// Offsets are 32-bit: one field (N rows of n elements) stays below 2^31.
+template <int N, int E>
+struct MakhoulRows {
+  int a, b, step;
+  __device__ __forceinline__ MakhoulRows(int q0, int c, int n)
+      : a(2 * q0 * n + c), b((2 * N - 1 - 2 * q0) * n + c), step(2 * (N / E) * n) {}
+  __device__ __forceinline__ int off(int e) const { return e < E / 2 ? a + e * step : b - e * step; }
+};
+
+// LDS slot fpad(q0 + e*T) of the same element: affine in e when T is a
+// multiple of 16 (fpad adds one slot per 16).
+template <int N, int E>
+__device__ __forceinline__ int slot_run(int q0, int e) {
+  constexpr int T = N / E;
+  if constexpr (T % 16 == 0)
+    return fpad(q0) + e * (T / 16 * 17);
+  else
+    return fpad(q0 + e * T);
+}
+
+// LDS slot fpad(dct_slot(t + e*T)) of row element t + e*T (T = N/E even): the
+// element's parity is t's, so its FFT slot runs by +-T/2 per e, and by
+// +-17T/32 after padding when T is a multiple of 32.
+template <int N, int E>
+__device__ __forceinline__ int row_slot(int t, int e) {
+  constexpr int T = N / E;
+  if constexpr (T % 32 == 0) {
+    constexpr int st = T / 32 * 17;
+    return fpad(dct_slot<N>(t)) + ((t & 1) ? -e * st : e * st);
+  } else {
+    return fpad(dct_slot<N>(t + e * T));
+  }
+}
+
+// Move a C-column tile between global memory (rows of stride n, Makhoul
+// order down the columns) and the LDS tile (column c at c * LS): element e of
+// thread tid is column tid % C, slot tid / C + e*T.
+template <int N, int E, int C, bool LOAD>
+__device__ __forceinline__ void col_tile_xfer(float2* lds, float2* base, int n, int tid) {
+  constexpr int LS = col_stride<N, C>();
+  const int c = tid % C, q0 = tid / C;
+  const MakhoulRows<N, E> rows(q0, c, n);
+  float2* l = lds + c * LS;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    if constexpr (LOAD)
+      l[slot_run<N, E>(q0, e)] = base[rows.off(e)];
+    else
+      base[rows.off(e)] = l[slot_run<N, E>(q0, e)];
+  }
+}
+
 }  // namespace htamd
