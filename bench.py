@@ -170,7 +170,7 @@ STAGES = ["prefilter", "rows_fwd", "cols", "rows_adj", "sadj", "prefilter_grad",
 # FETCH_SIZE and WRITE_SIZE runs scaled by what the counters read for a 256 MiB
 # device-to-device copy (reads x2.00, writes x1.00); the commands and per-kernel
 # figures are in profiles/bh_gradient_fused_2048_d4_a16_traffic.md.
-TRAFFIC_MIB = {"bh_fwd_rows_k": 1700.7 + 2048.0, "bh_cols_k": 3101.3 + 2640.7, "bh_adj_rows_k": 3983.1 + 2177.2}
+TRAFFIC_MIB = {"bh_fwd_rows_k": 1699.3 + 2048.0, "bh_cols_k": 3223.6 + 3380.4, "bh_adj_rows_k": 4004.4 + 2177.3}
 TRAFFIC_WORKLOAD = (2048, 4, 1024, 16)
 
 
