@@ -38,6 +38,7 @@ SIGNATURES = {
     "htamd_bh_gradient": [c_vp, c_i64] + [c_vp] * 8 + [c_int] + [c_vp] * 5 + [c_i64, c_vp],
     "htamd_bh_hessian_workspace_size": [c_vp, c_i64, c_int, ctypes.POINTER(c_i64)],
     "htamd_bh_hessian": [c_vp, c_i64] + [c_vp] * 11 + [c_dbl, c_vp, c_int, c_vp, c_i64, c_vp],
+    "htamd_bh_hessian_pair": [c_vp, c_i64] + [c_vp] * 11 + [c_dbl, c_vp, c_int, c_vp, c_i64, c_vp],
     "htamd_bspline_sback": [c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp],
     "htamd_bspline_coeffback": [c_vp, c_vp, c_vp, c_i64, c_dbl, c_vp],
     "htamd_axpby": [c_vp, c_vp, ctypes.c_float, ctypes.c_float, c_vp, c_i64, c_vp],

@@ -9,6 +9,9 @@
   sum (|D(prb exp(i S(coeff proj)))| - d)^2 / data_size of Rec (rec_mpi.py:517-527,
   815-835) whose backward is the fused BH gradient kernel (htamd_bh_gradient,
   rec_mpi.py:327-353 without the rho^2 scaling).
+* ``bh_bilinear(prb, proj, pos, data, rec, dir1, dir2=None)``: the BH bilinear
+  form of that objective (its second derivative along two directions) through
+  the fused bilinear-form kernel; the BH step sizes are ratios of these.
 
 For a real-valued loss of complex inputs PyTorch's gradient is
 dL/dx + i dL/dy, i.e. the g with dL = Re<g, dz> -- the convention of the
@@ -103,3 +106,18 @@ def bh_objective(prb, proj, pos, data, rec, st=0):
     gradient in backward."""
     _lib.require_gpu()
     return _BHObjective.apply(prb, proj, pos, data, rec, st)
+
+
+def bh_bilinear(prb, proj, pos, data, rec, dir1, dir2=None, st=0):
+    """Second derivative of the data term that bh_objective returns,
+    d^2 f(x)(y, z), through the fused bilinear-form kernel (htamd_bh_hessian,
+    Rec.hessian_cascade of rec_mpi.py:250-303): x = (prb, proj, pos), y = dir1
+    and z = dir2 (None: z = y), each a (dprb, dproj, dpos) triple.  The
+    reference's d2s pairing is kept, so for y != z this is the reference's
+    form, exact for y = z.  Returns a float64 device tensor [1]."""
+    _lib.require_gpu()
+    c = lambda t: t.detach().contiguous()  # noqa: E731
+    x = {"prb": c(prb), "proj": c(proj), "pos": c(pos)}
+    y = {"prb": c(dir1[0]), "proj": c(dir1[1]), "pos": c(dir1[2])}
+    z = y if dir2 is None else {"prb": c(dir2[0]), "proj": c(dir2[1]), "pos": c(dir2[2])}
+    return rec.hessian_fused(x, y, z, c(data), st)

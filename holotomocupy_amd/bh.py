@@ -73,8 +73,10 @@ class BHGradient:
                   _lib.ptr(ws), ws.numel(), _lib.stream_ptr())
 
     def hessian_workspace(self, nangles, same):
+        """same: False (general form), True (z = y) or 2 (the pair form)."""
         nb = ctypes.c_int64()
-        _lib.call("htamd_bh_hessian_workspace_size", self._plan, int(nangles), 1 if same else 0, ctypes.byref(nb))
+        mode = 2 if same == 2 and same is not True else (1 if same else 0)
+        _lib.call("htamd_bh_hessian_workspace_size", self._plan, int(nangles), mode, ctypes.byref(nb))
         if getattr(self, "_hws", None) is None or self._hws.numel() < nb.value:
             self._hws = None  # release the old one first
             self._hws = torch.empty(nb.value, dtype=torch.uint8, device=self.device)
@@ -100,4 +102,23 @@ class BHGradient:
                   _lib.ptr(x["prb"]), _lib.ptr(y["proj"]), _lib.ptr(y["pos"]), _lib.ptr(y["prb"]), *zz,
                   _lib.ptr(data), float(2.0 / self.rec.data_size), _lib.ptr(out), 1 if accumulate else 0,
                   _lib.ptr(ws), ws.numel(), _lib.stream_ptr())
+        return out
+
+    def hessian_pair(self, x, y, z, data, mag, out, accumulate=False):
+        """out[0] (+)= H(y, z), out[1] (+)= H(z, z) in one fused pass
+        (htamd_bh_hessian_pair): the two bilinear forms of the BH loop's beta
+        step (rec_mpi.py:186-188).  out: float64 device tensor [2]."""
+        B = x["proj"].shape[0]
+        parts = [x["proj"], x["pos"], mag, x["prb"], y["proj"], y["pos"], y["prb"], z["proj"], z["pos"], z["prb"]]
+        for t in parts + [data, out]:
+            if not t.is_contiguous() or t.device != self.device:
+                raise ValueError(f"bh hessian: every operand must be a contiguous tensor on {self.device}")
+        if out.numel() < 2:
+            raise ValueError("bh hessian pair: out needs two elements")
+        ws = self.hessian_workspace(B, 2)
+        _lib.call("htamd_bh_hessian_pair", self._plan, B, _lib.ptr(x["proj"]), _lib.ptr(x["pos"]), _lib.ptr(mag),
+                  _lib.ptr(x["prb"]), _lib.ptr(y["proj"]), _lib.ptr(y["pos"]), _lib.ptr(y["prb"]),
+                  _lib.ptr(z["proj"]), _lib.ptr(z["pos"]), _lib.ptr(z["prb"]), _lib.ptr(data),
+                  float(2.0 / self.rec.data_size), _lib.ptr(out), 1 if accumulate else 0, _lib.ptr(ws), ws.numel(),
+                  _lib.stream_ptr())
         return out

@@ -193,6 +193,22 @@ class BHLoop:
         """Bilinear form of the full functional (rec_mpi.py:235-248), as a float."""
         return float(self.hessian_dev(vars, grads, etas))
 
+    def hessian_pair_dev(self, vars, grads, etas):
+        """(hessian(vars, grads, etas), hessian(vars, etas, etas)) as a float64
+        device tensor [2]: the beta step's two forms, whose data terms share
+        one fused pass per chunk of angles."""
+        if not getattr(self, "fused_hessian", True):
+            return torch.cat([self.hessian_dev(vars, grads, etas), self.hessian_dev(vars, etas, etas)])
+        out = torch.zeros(2, dtype=torch.float64, device=self.device)
+        for st, B in self._chunks():
+            self.hessian_pair_fused(self._sl(vars, st, B), self._sl(grads, st, B), self._sl(etas, st, B),
+                                    self.data[st:st + B], st, out=out)
+        self._hessian_prbfit_dev(vars["prb"], grads["prb"], etas["prb"], out[0:1])
+        self._hessian_prbfit_dev(vars["prb"], etas["prb"], etas["prb"], out[1:2])
+        self.hessian_laplacian(grads["obj"], etas["obj"], out=out[0:1])
+        self.hessian_laplacian(etas["obj"], etas["obj"], out=out[1:2])
+        return out
+
     def min_dev(self, prb, obj, pos, proj):
         """Objective of Rec.min (rec_mpi.py:815-836) as a float64 device scalar."""
         out = self._scalar()
@@ -273,7 +289,8 @@ class BHLoop:
                 if i == self.start_iter:
                     beta = self._scalar()
                 else:
-                    beta = self.hessian_dev(vars, grads, etas) / self.hessian_dev(vars, etas, etas)
+                    tb = self.hessian_pair_dev(vars, grads, etas)
+                    beta = tb[0:1] / tb[1:2]
                 mark("hessian_beta")
                 ab = torch.cat([beta, self._scalar(-1.0)])
                 top = self._scalar()

@@ -46,7 +46,8 @@ struct BhhArgs {
   const float2* py = nullptr;
   const float2* pz = nullptr;
   const float* data = nullptr;  // [nang, ndist, nz, n]
-  float2* F[4] = {nullptr, nullptr, nullptr, nullptr};  // x, y, z, w fields [nang, ndist, nz, n]
+  // x, y, z, w fields and the second-order field of the (z, z) form (pair only), [nang, ndist, nz, n]
+  float2* F[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   double* part = nullptr;       // [column tiles]
   const float2* twx = nullptr;
   const float2* twy = nullptr;
@@ -57,13 +58,26 @@ struct BhhArgs {
 __device__ __forceinline__ float2 mul_i(float2 v) { return make_float2(-v.y, v.x); }
 
 // ---- H1: rows ---------------------------------------------------------------
+// MODE: kGeneral = H(y, z); kSame = H(y, y); kPair = H(y, z) and H(z, z) in
+// one pass (the beta step of the BH loop: they share the point and z).
+enum { kGeneral = 0, kSame = 1, kPair = 2 };
+
+template <int MODE>
+struct HessShape {
+  static constexpr bool SAME = MODE == kSame, PAIR = MODE == kPair;
+  static constexpr int NR = SAME ? 4 : PAIR ? 7 : 5;  // staged rows: R0, Ay, (Az), W1, W0, (W1zz, W0zz)
+  static constexpr int NF = SAME ? 3 : PAIR ? 5 : 4;  // fields: x, y, (z), w, (wzz)
+  static constexpr int NOUT = PAIR ? 2 : 1;
+};
+
 // rpb row units per workgroup (rpb * N/E threads, rpb <= kRowThreads / (N/E)),
 // chosen so that the staged rows fit the dynamic LDS
-template <int N, int E, typename TO, bool SAME>
+template <int N, int E, typename TO, int MODE>
 __global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
+  using HS = HessShape<MODE>;
+  constexpr bool SAME = HS::SAME, PAIR = HS::PAIR;
   constexpr int T = N / E, ROWS = kRowThreads / T;
-  constexpr int NR = SAME ? 4 : 5;  // staged rows per row unit: R0, Ay, (Az), W1, W0
-  constexpr int NF = SAME ? 3 : 4;  // fields: x, y, (z), w
+  constexpr int NR = HS::NR, NF = HS::NF;
   __shared__ float2 lds[ROWS * fft_buf<N>()];
   const int r = threadIdx.x / T, t0 = threadIdx.x % T;
   const int64_t nunits = (int64_t)a.ndist * a.nz;
@@ -79,8 +93,10 @@ __global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
   TO* R0 = rows;
   TO* Ay = rows + a.npsi;
   TO* Az = SAME ? Ay : rows + 2 * a.npsi;
-  TO* W1 = rows + (NR - 2) * a.npsi;
-  TO* W0 = rows + (NR - 1) * a.npsi;
+  TO* W1 = rows + (SAME ? 2 : 3) * a.npsi;
+  TO* W0 = rows + (SAME ? 3 : 4) * a.npsi;
+  TO* V1 = rows + 5 * a.npsi;  // pair: W1 and W0 of the (z, z) form
+  TO* V0 = rows + 6 * a.npsi;
   const int64_t prow = ((int64_t)j * a.nz + ty) * N;
   for (int64_t b = 0; b < a.nang; ++b) {
     int t = t0;
@@ -132,10 +148,14 @@ __global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
       if constexpr (!SAME) Az[o] = vfma(-zzy, r1, z0);
       W1[o] = vfma(-zzx, z0, vfma(-yyx, y0, vfma(cross, r1, vzero<TO>())));
       W0[o] = vfma(-zzy, z1, vfma(-yyy, y1, vfma(yyy * zzy, r2, vzero<TO>())));
+      if constexpr (PAIR) {
+        V1[o] = vfma(-2.0f * zzx, z0, vfma(2.0f * zzx * zzy, r1, vzero<TO>()));
+        V0[o] = vfma(-2.0f * zzy, z1, vfma(zzy * zzy, r2, vzero<TO>()));
+      }
     }
     __syncthreads();
     // per-pixel values of the second-order chain, kept for the NF transforms
-    float2 ev[E], y12[E], z12[SAME ? 1 : E], w12[E];
+    float2 ev[E], y12[E], z12[SAME ? 1 : E], w12[E], v12[PAIR ? E : 1];
     auto chain = [&](int e) {
       const int i = t + e * T;
       const float x = bs_coord(i, N, a.npsi, mb, rx);
@@ -146,7 +166,7 @@ __global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
       bs_dw(x - fx, dwx);
       bs_d2w(x - fx, d2wx);
       TO x22 = vzero<TO>(), sx = vzero<TO>(), sxx = vzero<TO>(), ay = vzero<TO>(), az = vzero<TO>();
-      TO w1 = vzero<TO>(), w0 = vzero<TO>();
+      TO w1 = vzero<TO>(), w0 = vzero<TO>(), v1 = vzero<TO>(), v0 = vzero<TO>();
 #pragma unroll
       for (int jx = 0; jx < 4; ++jx) {
         const int xx = ix + jx;
@@ -160,6 +180,10 @@ __global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
         if constexpr (!SAME) az = vfma(wx[jx], Az[o], az);
         w1 = vfma(dwx[jx], W1[o], w1);
         w0 = vfma(wx[jx], W0[o], w0);
+        if constexpr (PAIR) {
+          v1 = vfma(dwx[jx], V1[o], v1);
+          v0 = vfma(wx[jx], V0[o], v0);
+        }
       }
       const float2 yv = to_c(vfma(-yyx, sx, ay));
       const float2 wv = to_c(vfma(yyx * zzx, sxx, vfma(1.0f, w1, w0)));
@@ -174,10 +198,16 @@ __global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
         z12[e] = cmul(ex, mul_i(zv));
         const float2 yz = cmul(yv, zv);
         w12[e] = cmul(ex, make_float2(-wv.y - yz.x, wv.x - yz.y));
+        if constexpr (PAIR) {
+          const float2 vv = to_c(vfma(zzx * zzx, sxx, vfma(1.0f, v1, v0)));
+          const float2 zz = cmul(zv, zv);
+          v12[e] = cmul(ex, make_float2(-vv.y - zz.x, vv.x - zz.y));
+        }
       }
     };
     // equal directions: full unroll keeps the three arrays in registers; the
-    // general form's four arrays measure faster in scratch with a compact loop
+    // general and pair forms' four or five arrays measure faster in scratch
+    // with a compact loop
     if constexpr (SAME) {
 #pragma unroll
       for (int e = 0; e < E; ++e) chain(e);
@@ -185,10 +215,10 @@ __global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
 #pragma unroll 2
       for (int e = 0; e < E; ++e) chain(e);
     }
-    // fields in the order w, y, z, x, one LDS transform each
+    // fields (slots x 0, y 1, z 2, w 3, w of (z, z) 4) in the order w, (wzz,) y, (z,) x
 #pragma unroll 1
     for (int k = 0; k < NF; ++k) {
-      const int kk = k == 0 ? 3 : k == NF - 1 ? 0 : k;  // field slot: x 0, y 1, z 2, w 3
+      const int kk = k == 0 ? 3 : k == NF - 1 ? 0 : PAIR ? (k == 1 ? 4 : k - 1) : k;
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int i = t + e * T;
@@ -200,12 +230,14 @@ __global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
           v = cadd(cmul(a.py[prow + i], ev[e]), cmul(p, y12[e]));
         } else if (kk == 2) {
           v = cadd(cmul(a.pz[prow + i], ev[e]), cmul(p, z12[SAME ? 0 : e]));
-        } else {
+        } else if (kk == 3) {
           const float2 pyv = a.py[prow + i];
           if constexpr (SAME)
             v = cadd(cscale(cmul(pyv, y12[e]), 2.0f), cmul(p, w12[e]));
           else
             v = cadd(cadd(cmul(pyv, z12[e]), cmul(a.pz[prow + i], y12[e])), cmul(p, w12[e]));
+        } else {
+          v = cadd(cscale(cmul(a.pz[prow + i], z12[SAME ? 0 : e]), 2.0f), cmul(p, v12[PAIR ? e : 0]));
         }
         s[row_slot<N, E>(t, e)] = v;
       }
@@ -234,11 +266,38 @@ __device__ __forceinline__ void hcol_transform(float2* lds, const float2* base, 
   fresnel_1d<N, E>(lds + cc * LS, t, twr, tab);
 }
 
-template <int N, int E, int C, bool SAME>
+template <int NT>
+__device__ __forceinline__ double block_sum_d(double v, double* red) {
+  if constexpr (NT >= 64) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double tot = 0.0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 0; k < NT / 64; ++k) tot += red[k];
+    }
+    __syncthreads();
+    return tot;
+  } else {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    double tot = 0.0;
+    if (threadIdx.x == 0)
+      for (int k = 0; k < NT; ++k) tot += red[k];
+    __syncthreads();
+    return tot;
+  }
+}
+
+template <int N, int E, int C, int MODE>
 __global__ __launch_bounds__(C*(N / E)) void bhh_cols_k(BhhArgs a, int tiles_per_field) {
+  using HS = HessShape<MODE>;
+  constexpr bool SAME = HS::SAME, PAIR = HS::PAIR;
   constexpr int T = N / E, NT = C * T, LS = col_stride<N, C>();
   __shared__ float2 lds[C * LS];
-  __shared__ double red[NT / 64 > 0 ? NT / 64 : 1];
+  __shared__ double red[NT >= 64 ? NT / 64 : NT];
   const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const int64_t f = tile / tiles_per_field;  // field b*ndist + j
   const int x0c = (int)(tile % tiles_per_field) * C;
@@ -259,8 +318,8 @@ __global__ __launch_bounds__(C*(N / E)) void bhh_cols_k(BhhArgs a, int tiles_per
     dv[e] = a.data[off + (int64_t)dct_nat<N>(t + e * T) * a.n + cc];
   }
   __syncthreads();
-  // (1 - d/|x0|) Re(conj(x0) w0)
-  double acc = 0.0;
+  // (1 - d/|x0|) Re(conj(x0) w0), for each form
+  double acc = 0.0, acc2 = 0.0;
   hcol_transform<N, E, C>(lds, a.F[3] + off, a.n, t, cc, twr, tab);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -269,6 +328,16 @@ __global__ __launch_bounds__(C*(N / E)) void bhh_cols_k(BhhArgs a, int tiles_per
     acc += (double)((1.0f - dv[e] / am) * (xv[e].x * w.x + xv[e].y * w.y));
   }
   __syncthreads();
+  if constexpr (PAIR) {
+    hcol_transform<N, E, C>(lds, a.F[4] + off, a.n, t, cc, twr, tab);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const float2 w = s[fpad(t + e * T)];
+      const float am = sqrtf(xv[e].x * xv[e].x + xv[e].y * xv[e].y);
+      acc2 += (double)((1.0f - dv[e] / am) * (xv[e].x * w.x + xv[e].y * w.y));
+    }
+    __syncthreads();
+  }
   hcol_transform<N, E, C>(lds, a.F[1] + off, a.n, t, cc, twr, tab);
   if constexpr (SAME) {
 #pragma unroll
@@ -290,48 +359,41 @@ __global__ __launch_bounds__(C*(N / E)) void bhh_cols_k(BhhArgs a, int tiles_per
     for (int e = 0; e < E; ++e) {
       const float2 zv = s[fpad(t + e * T)];
       const float am = sqrtf(xv[e].x * xv[e].x + xv[e].y * xv[e].y);
+      const float g = 1.0f - dv[e] / am, h = dv[e] / (am * am * am);
       const float xy = xv[e].x * yv[e].x + xv[e].y * yv[e].y;
       const float xz = xv[e].x * zv.x + xv[e].y * zv.y;
-      const float t1 = (1.0f - dv[e] / am) * (yv[e].x * zv.x + yv[e].y * zv.y);
-      const float t2 = dv[e] / (am * am * am) * xy * xz;
-      acc += (double)(t1 + t2);
+      acc += (double)(g * (yv[e].x * zv.x + yv[e].y * zv.y) + h * xy * xz);
+      if constexpr (PAIR) acc2 += (double)(g * (zv.x * zv.x + zv.y * zv.y) + h * xz * xz);
     }
   }
-  if constexpr (NT >= 64) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-    __syncthreads();
+  const double tot = block_sum_d<NT>(acc, red);
+  if constexpr (PAIR) {
+    const double tot2 = block_sum_d<NT>(acc2, red);
     if (threadIdx.x == 0) {
-      double tot = 0.0;
-#pragma unroll
-      for (int k = 0; k < NT / 64; ++k) tot += red[k];
-      a.part[tile] = tot;
+      a.part[2 * tile] = tot;
+      a.part[2 * tile + 1] = tot2;
     }
   } else {
-    __shared__ double all[NT];
-    all[threadIdx.x] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double tot = 0.0;
-      for (int k = 0; k < NT; ++k) tot += all[k];
-      a.part[tile] = tot;
-    }
+    if (threadIdx.x == 0) a.part[tile] = tot;
   }
 }
 
-__global__ __launch_bounds__(256) void bhh_sum_k(const double* __restrict__ part, int64_t count, double scale,
-                                                 int accumulate, double* __restrict__ out) {
+// out[k] (+)= scale * sum over tiles t of part[t * nout + k], fixed order
+__global__ __launch_bounds__(256) void bhh_sum_k(const double* __restrict__ part, int64_t count, int nout,
+                                                 double scale, int accumulate, double* __restrict__ out) {
   __shared__ double r[256];
-  double s = 0.0;
-  for (int64_t i = threadIdx.x; i < count; i += 256) s += part[i];
-  r[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) r[threadIdx.x] += r[threadIdx.x + w];
+  for (int k = 0; k < nout; ++k) {
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < count; i += 256) s += part[i * nout + k];
+    r[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) r[threadIdx.x] += r[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = accumulate ? out[k] + r[0] * scale : r[0] * scale;
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = accumulate ? out[0] + r[0] * scale : r[0] * scale;
 }
 
 // ---- host side --------------------------------------------------------------
@@ -364,11 +426,12 @@ static int hess_cols_c(int64_t nz) {
 }
 
 struct BhhLayout {
-  int64_t c = 0, cy = 0, cz = 0, F[4] = {0, 0, 0, 0}, part = 0, total = 0;
+  int64_t c = 0, cy = 0, cz = 0, F[5] = {0, 0, 0, 0, 0}, part = 0, total = 0;
 };
 
-static BhhLayout bhh_layout(const htamd_prop_plan* pp, const htamd_shift_plan* sp, int64_t nang, bool same) {
+static BhhLayout bhh_layout(const htamd_prop_plan* pp, const htamd_shift_plan* sp, int64_t nang, int mode) {
   BhhLayout L;
+  const bool same = mode == kSame, pair = mode == kPair;
   const int64_t osz = sp->is_complex ? 8 : 4;
   const int64_t cbytes = align256(nang * sp->nzpsi * sp->npsi * osz);
   const int64_t fbytes = align256(nang * pp->ndist * pp->n * pp->nz * 8);
@@ -379,9 +442,9 @@ static BhhLayout bhh_layout(const htamd_prop_plan* pp, const htamd_shift_plan* s
   off += cbytes;
   L.cz = same ? L.cy : off;
   if (!same) off += cbytes;
-  for (int k = 0; k < 4; ++k) {
-    if (same && k == 2) {
-      L.F[2] = L.F[1];
+  for (int k = 0; k < 5; ++k) {
+    if ((same && k == 2) || (!pair && k == 4)) {
+      L.F[k] = L.F[same && k == 2 ? 1 : 3];
       continue;
     }
     L.F[k] = off;
@@ -390,7 +453,7 @@ static BhhLayout bhh_layout(const htamd_prop_plan* pp, const htamd_shift_plan* s
   const int C = hess_cols_c(pp->nz);
   const int64_t ntiles = nang * pp->ndist * (pp->n / C);
   L.part = off;
-  off += align256(ntiles * 8);
+  off += align256(ntiles * 8 * (pair ? 2 : 1));
   L.total = off;
   return L;
 }
@@ -399,69 +462,75 @@ static BhhLayout bhh_layout(const htamd_prop_plan* pp, const htamd_shift_plan* s
 // rows of a workgroup stay within 64 KiB of dynamic LDS.
 constexpr int64_t kHessDynBudget = 65536;
 
-template <int N, typename TO>
-static hipError_t hess_rows(bool same, const BhhArgs& a, hipStream_t st) {
+template <int N, typename TO, int MODE>
+static hipError_t hess_rows_m(const BhhArgs& a, hipStream_t st) {
   constexpr int E = N >= 8 ? 8 : N;
   constexpr int T = N / E, ROWS = kRowThreads / T;
-  const int64_t per_row = (int64_t)(same ? 4 : 5) * a.npsi * (int64_t)sizeof(TO);
+  const int64_t per_row = (int64_t)HessShape<MODE>::NR * a.npsi * (int64_t)sizeof(TO);
   int rpb = ROWS;
   while (rpb > 1 && rpb * per_row > kHessDynBudget) rpb >>= 1;
   if (rpb * per_row > kHessDynBudget) return hipErrorInvalidConfiguration;
   const int64_t grid = ((int64_t)a.ndist * a.nz + rpb - 1) / rpb;
-  const size_t dyn = (size_t)(rpb * per_row);
-  if (same)
-    hipLaunchKernelGGL((bhh_rows_k<N, E, TO, true>), dim3((unsigned)grid), dim3(rpb * T), dyn, st, a, rpb);
-  else
-    hipLaunchKernelGGL((bhh_rows_k<N, E, TO, false>), dim3((unsigned)grid), dim3(rpb * T), dyn, st, a, rpb);
+  hipLaunchKernelGGL((bhh_rows_k<N, E, TO, MODE>), dim3((unsigned)grid), dim3(rpb * T), (size_t)(rpb * per_row),
+                     st, a, rpb);
   return hipGetLastError();
 }
 
+template <int N, typename TO>
+static hipError_t hess_rows(int mode, const BhhArgs& a, hipStream_t st) {
+  if (mode == kSame) return hess_rows_m<N, TO, kSame>(a, st);
+  if (mode == kPair) return hess_rows_m<N, TO, kPair>(a, st);
+  return hess_rows_m<N, TO, kGeneral>(a, st);
+}
+
 template <typename TO>
-static hipError_t hess_rows_dispatch(bool same, const BhhArgs& a, hipStream_t st) {
+static hipError_t hess_rows_dispatch(int mode, const BhhArgs& a, hipStream_t st) {
   switch (a.n) {
-    case 8: return hess_rows<8, TO>(same, a, st);
-    case 16: return hess_rows<16, TO>(same, a, st);
-    case 32: return hess_rows<32, TO>(same, a, st);
-    case 64: return hess_rows<64, TO>(same, a, st);
-    case 128: return hess_rows<128, TO>(same, a, st);
-    case 256: return hess_rows<256, TO>(same, a, st);
-    case 512: return hess_rows<512, TO>(same, a, st);
-    case 1024: return hess_rows<1024, TO>(same, a, st);
-    case 2048: return hess_rows<2048, TO>(same, a, st);
+    case 8: return hess_rows<8, TO>(mode, a, st);
+    case 16: return hess_rows<16, TO>(mode, a, st);
+    case 32: return hess_rows<32, TO>(mode, a, st);
+    case 64: return hess_rows<64, TO>(mode, a, st);
+    case 128: return hess_rows<128, TO>(mode, a, st);
+    case 256: return hess_rows<256, TO>(mode, a, st);
+    case 512: return hess_rows<512, TO>(mode, a, st);
+    case 1024: return hess_rows<1024, TO>(mode, a, st);
+    case 2048: return hess_rows<2048, TO>(mode, a, st);
     default: return hipErrorInvalidValue;
   }
 }
 
 template <int NZ>
-static hipError_t hess_cols(bool same, const BhhArgs& a, hipStream_t st) {
+static hipError_t hess_cols(int mode, const BhhArgs& a, hipStream_t st) {
   constexpr int E = bhh_col_e<NZ>(), C = bhh_cols_c<NZ>();
   if (a.n % C) return hipErrorInvalidValue;
   const int tpf = a.n / C;
-  const int64_t grid = a.nang * a.ndist * tpf;
-  if (same)
-    hipLaunchKernelGGL((bhh_cols_k<NZ, E, C, true>), dim3((unsigned)grid), dim3(C * (NZ / E)), 0, st, a, tpf);
+  const dim3 grid((unsigned)(a.nang * a.ndist * tpf)), block(C * (NZ / E));
+  if (mode == kSame)
+    hipLaunchKernelGGL((bhh_cols_k<NZ, E, C, kSame>), grid, block, 0, st, a, tpf);
+  else if (mode == kPair)
+    hipLaunchKernelGGL((bhh_cols_k<NZ, E, C, kPair>), grid, block, 0, st, a, tpf);
   else
-    hipLaunchKernelGGL((bhh_cols_k<NZ, E, C, false>), dim3((unsigned)grid), dim3(C * (NZ / E)), 0, st, a, tpf);
+    hipLaunchKernelGGL((bhh_cols_k<NZ, E, C, kGeneral>), grid, block, 0, st, a, tpf);
   return hipGetLastError();
 }
 
-static hipError_t hess_cols_dispatch(bool same, const BhhArgs& a, hipStream_t st) {
+static hipError_t hess_cols_dispatch(int mode, const BhhArgs& a, hipStream_t st) {
   switch (a.nz) {
-    case 8: return hess_cols<8>(same, a, st);
-    case 16: return hess_cols<16>(same, a, st);
-    case 32: return hess_cols<32>(same, a, st);
-    case 64: return hess_cols<64>(same, a, st);
-    case 128: return hess_cols<128>(same, a, st);
-    case 256: return hess_cols<256>(same, a, st);
-    case 512: return hess_cols<512>(same, a, st);
-    case 1024: return hess_cols<1024>(same, a, st);
-    case 2048: return hess_cols<2048>(same, a, st);
+    case 8: return hess_cols<8>(mode, a, st);
+    case 16: return hess_cols<16>(mode, a, st);
+    case 32: return hess_cols<32>(mode, a, st);
+    case 64: return hess_cols<64>(mode, a, st);
+    case 128: return hess_cols<128>(mode, a, st);
+    case 256: return hess_cols<256>(mode, a, st);
+    case 512: return hess_cols<512>(mode, a, st);
+    case 1024: return hess_cols<1024>(mode, a, st);
+    case 2048: return hess_cols<2048>(mode, a, st);
     default: return hipErrorInvalidValue;
   }
 }
 
 template <typename TO>
-static hipError_t bhh_run(const htamd_shift_plan* sp, bool same, BhhArgs a, const void* proj, const void* projy,
+static hipError_t bhh_run(const htamd_shift_plan* sp, int mode, BhhArgs a, const void* proj, const void* projy,
                           const void* projz, char* ws, const BhhLayout& L, double scale, int accumulate,
                           double* result, hipStream_t st) {
   hipError_t e;
@@ -471,14 +540,15 @@ static hipError_t bhh_run(const htamd_shift_plan* sp, bool same, BhhArgs a, cons
   if ((e = bs_launch_prefilter<TO>((const TO*)projy, (TO*)(ws + L.cy), a.nang, a.nzpsi, a.npsi, sp->taps, st)) !=
       hipSuccess)
     return e;
-  if (!same &&
+  if (mode != kSame &&
       (e = bs_launch_prefilter<TO>((const TO*)projz, (TO*)(ws + L.cz), a.nang, a.nzpsi, a.npsi, sp->taps, st)) !=
           hipSuccess)
     return e;
-  if ((e = hess_rows_dispatch<TO>(same, a, st)) != hipSuccess) return e;
-  if ((e = hess_cols_dispatch(same, a, st)) != hipSuccess) return e;
+  if ((e = hess_rows_dispatch<TO>(mode, a, st)) != hipSuccess) return e;
+  if ((e = hess_cols_dispatch(mode, a, st)) != hipSuccess) return e;
   const int64_t ntiles = a.nang * a.ndist * (a.n / hess_cols_c(a.nz));
-  hipLaunchKernelGGL(bhh_sum_k, dim3(1), dim3(256), 0, st, a.part, ntiles, scale, accumulate, result);
+  hipLaunchKernelGGL(bhh_sum_k, dim3(1), dim3(256), 0, st, a.part, ntiles, mode == kPair ? 2 : 1, scale,
+                     accumulate, result);
   return hipGetLastError();
 }
 
@@ -490,37 +560,36 @@ extern "C" {
 
 int htamd_bh_hessian_workspace_size(const htamd_bh_plan* p, int64_t nangles, int same_directions,
                                     int64_t* bytes) {
-  if (!p || !bytes || nangles < 0) return fail(HTAMD_ERR_ARG, "bh_hessian_workspace_size: bad arguments");
-  *bytes = bhh_layout(p->prop, p->shift, nangles, same_directions != 0).total;
+  if (!p || !bytes || nangles < 0 || same_directions < 0 || same_directions > 2)
+    return fail(HTAMD_ERR_ARG, "bh_hessian_workspace_size: bad arguments");
+  *bytes = bhh_layout(p->prop, p->shift, nangles, same_directions).total;
   return HTAMD_OK;
 }
 
-int htamd_bh_hessian(const htamd_bh_plan* p, int64_t nangles, const void* proj, const float* pos, const float* mag,
-                     const void* prb, const void* proj_y, const float* pos_y, const void* prb_y, const void* proj_z,
-                     const float* pos_z, const void* prb_z, const float* data, double scale, double* result,
-                     int accumulate, void* workspace, int64_t workspace_bytes, void* stream) {
-  if (!p) return fail(HTAMD_ERR_ARG, "bh_hessian: plan is NULL");
-  if (nangles < 0 || !result) return fail(HTAMD_ERR_ARG, "bh_hessian: bad arguments");
-  const bool same = proj_z == nullptr && pos_z == nullptr && prb_z == nullptr;
+static int bh_hessian_impl(const htamd_bh_plan* p, int mode, int64_t nangles, const void* proj, const float* pos,
+                           const float* mag, const void* prb, const void* proj_y, const float* pos_y,
+                           const void* prb_y, const void* proj_z, const float* pos_z, const void* prb_z,
+                           const float* data, double scale, double* result, int accumulate, void* workspace,
+                           int64_t workspace_bytes, void* stream, const char* what) {
+  const int nout = mode == kPair ? 2 : 1;
   if (nangles == 0) {
     if (!accumulate) {
-      hipLaunchKernelGGL(bhh_sum_k, dim3(1), dim3(256), 0, (hipStream_t)stream, (const double*)nullptr, 0, 0.0,
-                         0, result);
+      hipLaunchKernelGGL(bhh_sum_k, dim3(1), dim3(256), 0, (hipStream_t)stream, (const double*)nullptr, 0, nout,
+                         0.0, 0, result);
       hipError_t e = hipGetLastError();
-      if (e != hipSuccess) return hip_fail(e, "bh_hessian");
+      if (e != hipSuccess) return hip_fail(e, what);
     }
     return HTAMD_OK;
   }
   if (!proj || !pos || !mag || !prb || !proj_y || !pos_y || !prb_y || !data)
-    return fail(HTAMD_ERR_ARG, "bh_hessian: NULL buffer");
-  if (!same && (!proj_z || !pos_z || !prb_z))
-    return fail(HTAMD_ERR_ARG, "bh_hessian: give all three parts of direction z, or none (z = y)");
-  const BhhLayout L = bhh_layout(p->prop, p->shift, nangles, same);
+    return fail(HTAMD_ERR_ARG, std::string(what) + ": NULL buffer");
+  const BhhLayout L = bhh_layout(p->prop, p->shift, nangles, mode);
   if (!workspace || workspace_bytes < L.total)
-    return fail(HTAMD_ERR_ARG, "bh_hessian: workspace of " + std::to_string(L.total) + " bytes required");
+    return fail(HTAMD_ERR_ARG, std::string(what) + ": workspace of " + std::to_string(L.total) + " bytes required");
   int dev = -1;
   (void)hipGetDevice(&dev);
-  if (dev != p->prop->device) return fail(HTAMD_ERR_ARG, "bh_hessian: plan belongs to another device");
+  if (dev != p->prop->device) return fail(HTAMD_ERR_ARG, std::string(what) + ": plan belongs to another device");
+  const bool same = mode == kSame;
   char* ws = (char*)workspace;
   BhhArgs a;
   a.n = (int)p->prop->n;
@@ -540,20 +609,47 @@ int htamd_bh_hessian(const htamd_bh_plan* p, int64_t nangles, const void* proj, 
   a.py = (const float2*)prb_y;
   a.pz = (const float2*)(same ? prb_y : prb_z);
   a.data = data;
-  for (int k = 0; k < 4; ++k) a.F[k] = (float2*)(ws + L.F[k]);
+  static_assert(sizeof(a.F) / sizeof(a.F[0]) == sizeof(L.F) / sizeof(L.F[0]), "one pointer per field slot");
+  for (int k = 0; k < 5; ++k) a.F[k] = (float2*)(ws + L.F[k]);
   a.part = (double*)(ws + L.part);
   a.twx = p->prop->tw_x;
   a.twy = p->prop->tw_y;
   a.tabx = p->prop->tab_x;
   a.taby = p->prop->tab_y;
   hipError_t e = p->shift->is_complex
-      ? bhh_run<float2>(p->shift, same, a, proj, proj_y, proj_z, ws, L, scale, accumulate, result,
+      ? bhh_run<float2>(p->shift, mode, a, proj, proj_y, proj_z, ws, L, scale, accumulate, result,
                         (hipStream_t)stream)
-      : bhh_run<float>(p->shift, same, a, proj, proj_y, proj_z, ws, L, scale, accumulate, result,
+      : bhh_run<float>(p->shift, mode, a, proj, proj_y, proj_z, ws, L, scale, accumulate, result,
                        (hipStream_t)stream);
   if (e == hipErrorInvalidConfiguration)
-    return fail(HTAMD_ERR_UNSUPPORTED, "bh_hessian: the staged object rows of one row pass exceed the LDS");
-  return e == hipSuccess ? HTAMD_OK : hip_fail(e, "bh_hessian");
+    return fail(HTAMD_ERR_UNSUPPORTED, std::string(what) + ": the staged object rows of one row pass exceed the LDS");
+  return e == hipSuccess ? HTAMD_OK : hip_fail(e, what);
+}
+
+int htamd_bh_hessian(const htamd_bh_plan* p, int64_t nangles, const void* proj, const float* pos, const float* mag,
+                     const void* prb, const void* proj_y, const float* pos_y, const void* prb_y, const void* proj_z,
+                     const float* pos_z, const void* prb_z, const float* data, double scale, double* result,
+                     int accumulate, void* workspace, int64_t workspace_bytes, void* stream) {
+  if (!p) return fail(HTAMD_ERR_ARG, "bh_hessian: plan is NULL");
+  if (nangles < 0 || !result) return fail(HTAMD_ERR_ARG, "bh_hessian: bad arguments");
+  const bool same = proj_z == nullptr && pos_z == nullptr && prb_z == nullptr;
+  if (!same && (!proj_z || !pos_z || !prb_z))
+    return fail(HTAMD_ERR_ARG, "bh_hessian: give all three parts of direction z, or none (z = y)");
+  return bh_hessian_impl(p, same ? kSame : kGeneral, nangles, proj, pos, mag, prb, proj_y, pos_y, prb_y, proj_z,
+                         pos_z, prb_z, data, scale, result, accumulate, workspace, workspace_bytes, stream,
+                         "bh_hessian");
+}
+
+int htamd_bh_hessian_pair(const htamd_bh_plan* p, int64_t nangles, const void* proj, const float* pos,
+                          const float* mag, const void* prb, const void* proj_y, const float* pos_y,
+                          const void* prb_y, const void* proj_z, const float* pos_z, const void* prb_z,
+                          const float* data, double scale, double* result, int accumulate, void* workspace,
+                          int64_t workspace_bytes, void* stream) {
+  if (!p) return fail(HTAMD_ERR_ARG, "bh_hessian_pair: plan is NULL");
+  if (nangles < 0 || !result || (nangles > 0 && (!proj_z || !pos_z || !prb_z)))
+    return fail(HTAMD_ERR_ARG, "bh_hessian_pair: bad arguments");
+  return bh_hessian_impl(p, kPair, nangles, proj, pos, mag, prb, proj_y, pos_y, prb_y, proj_z, pos_z, prb_z, data,
+                         scale, result, accumulate, workspace, workspace_bytes, stream, "bh_hessian_pair");
 }
 
 }  // extern "C"

@@ -331,6 +331,20 @@ class Rec(BHLoop):
         self._bh.hessian(vars, grads, z, data, mag, res, accumulate=True)
         return res
 
+    def hessian_pair_fused(self, vars, grads, etas, data=None, st=0, out=None):
+        """(H(grads, etas), H(etas, etas)) of the data term in one fused pass
+        (htamd_bh_hessian_pair), added into out (float64 device tensor [2],
+        zeros if not given) and returned."""
+        from .bh import BHGradient
+        data = self.data if data is None else data
+        B = vars["proj"].shape[0]
+        if getattr(self, "_bh", None) is None:
+            self._bh = BHGradient(self)
+        mag = self.eff_demagnifications[st:st + B].contiguous()
+        res = torch.zeros(2, dtype=torch.float64, device=self.device) if out is None else out
+        self._bh.hessian_pair(vars, grads, etas, data, mag, res, accumulate=True)
+        return res
+
     def hessian_cascade(self, vars, grads, etas, data=None, st=0):
         """BH bilinear form of the data term (rec_mpi.py:250-303), as a float."""
         return float(self._hessian_cascade_dev(vars, grads, etas, data, st))

@@ -232,7 +232,8 @@ int htamd_bh_gradient(const htamd_bh_plan* plan, int64_t nangles, const void* pr
  * reference's y_is_z branch: three propagations instead of four).  The second-order shift term keeps
  * the reference's d2s pairing (cuda_kernels.py:377-450).  Layouts as htamd_bh_gradient (one probe per
  * distance); result is one device double.  Deterministic (fixed-order FP64 partial sums).  The
- * workspace holds the prefiltered projections and the propagated fields of the chunk. */
+ * workspace holds the prefiltered projections and the propagated fields of the chunk; its size
+ * takes same_directions = 0 (general), 1 (z = y) or 2 (the pair form below). */
 int htamd_bh_hessian_workspace_size(const htamd_bh_plan* plan, int64_t nangles, int same_directions,
                                     int64_t* bytes);
 int htamd_bh_hessian(const htamd_bh_plan* plan, int64_t nangles, const void* proj, const float* pos,
@@ -240,6 +241,15 @@ int htamd_bh_hessian(const htamd_bh_plan* plan, int64_t nangles, const void* pro
                      const void* prb_y, const void* proj_z, const float* pos_z, const void* prb_z,
                      const float* data, double scale, double* result, int accumulate, void* workspace,
                      int64_t workspace_bytes, void* stream);
+/* The two bilinear forms of the BH loop's beta step, H(y, z) and H(z, z) (rec_mpi.py:186-188: top =
+ * hessian(vars, grads, etas), bottom = hessian(vars, etas, etas)), in one pass: result[0] (+)= H(y, z),
+ * result[1] (+)= H(z, z); five propagated fields serve both instead of seven.  No reference
+ * counterpart as one call.  Workspace: htamd_bh_hessian_workspace_size with same_directions = 2. */
+int htamd_bh_hessian_pair(const htamd_bh_plan* plan, int64_t nangles, const void* proj, const float* pos,
+                          const float* mag, const void* prb, const void* proj_y, const float* pos_y,
+                          const void* prb_y, const void* proj_z, const float* pos_z, const void* prb_z,
+                          const float* data, double scale, double* result, int accumulate, void* workspace,
+                          int64_t workspace_bytes, void* stream);
 
 /* ---- BLAS-1 helpers of the BH iteration ----------------------------------------
  * Arrays are float streams of n floats (a complex64 array of m elements is

@@ -54,3 +54,41 @@ def test_linear_operator_backward_is_adjoint(gpu):
     L = torch.sum(torch.real(shift(c, r, m, sh) * torch.conj(q)))
     (gc,) = torch.autograd.grad(L, [c])
     assert torch.allclose(gc, sh.Sadj(q, r, m), rtol=1e-6, atol=1e-6)
+
+
+def test_bh_bilinear_is_derivative_of_autograd_gradient(gpu):
+    """bh_bilinear(x; v, v) against central differences (with Richardson
+    extrapolation) of <grad bh_objective(x + t v), v> at t = 0, the gradient
+    coming from autograd; and against the oracle's bilinear form for v != w.
+    The differences of FP32 gradients at step 0.05 carry about 2e-3 of
+    rounding (the FP64 oracle's own truncation error there is 6e-6), so that
+    check is at 5e-3; the oracle check is the 1e-4 one."""
+    from holotomocupy_amd.autograd import bh_bilinear, bh_objective
+    print("[case] autograd bh_bilinear", flush=True)
+    rec, cas, x, v, w, d = make_problem(False, seed=62)
+    xd, vd, wd = dev(x), dev(v), dev(w)
+    dd = torch.from_numpy(d).cuda()
+    keys = ("prb", "proj", "pos")
+
+    def dgrad(t):
+        xs = [(xd[k] + t * vd[k]).requires_grad_(True) for k in keys]
+        gs = torch.autograd.grad(bh_objective(*xs, dd, rec), xs)
+        tot = 0.0
+        for g, k in zip(gs, keys):
+            if g.is_complex():
+                tot += float(torch.sum(torch.real(torch.conj(g.to(torch.complex128)) * vd[k].to(torch.complex128))))
+            else:
+                tot += float(torch.sum(g.double() * vd[k].double()))
+        return tot
+
+    def central(h):
+        return (dgrad(h) - dgrad(-h)) / (2 * h)
+
+    h = 0.05
+    fd = (4 * central(h / 2) - central(h)) / 3
+    hvv = float(bh_bilinear(xd["prb"], xd["proj"], xd["pos"], dd, rec, (vd["prb"], vd["proj"], vd["pos"])))
+    print("bilinear", hvv, "central differences", fd, flush=True)
+    assert hvv == pytest.approx(fd, rel=5e-3)
+    hvw = float(bh_bilinear(xd["prb"], xd["proj"], xd["pos"], dd, rec, (vd["prb"], vd["proj"], vd["pos"]),
+                            (wd["prb"], wd["proj"], wd["pos"])))
+    assert hvw == pytest.approx(cas.hessian(lst(x), lst(v), lst(w), d), rel=1e-4)

@@ -78,3 +78,20 @@ def test_fused_hessian_bench_size_properties(gpu):
     h_plain = rec.hessian_cascade(x, v, w, data)
     print("same", h_same, "copy", h_copy, "vw", h_vw, "wv", h_wv, "plain", h_plain, flush=True)
     assert h_vw == pytest.approx(h_plain, rel=1e-4)
+
+
+@pytest.mark.parametrize("real", [False, True])
+def test_pair_form_matches_the_two_forms(gpu, real):
+    """htamd_bh_hessian_pair: (H(v, w), H(w, w)) in one pass equals the two
+    separate fused forms (same arithmetic, regrouped: 1e-5) and the oracle."""
+    print(f"[case] fused hessian pair real={real}", flush=True)
+    rec, cas, x, v, w, d = make_problem(real, seed=10)
+    xd, vd, wd, dd = dev(x), dev(v), dev(w), torch.from_numpy(d).cuda()
+    pair = rec.hessian_pair_fused(xd, vd, wd, dd)
+    h_vw = float(rec.hessian_fused(xd, vd, wd, dd))
+    h_ww = float(rec.hessian_fused(xd, wd, wd, dd))
+    assert float(pair[0]) == pytest.approx(h_vw, rel=1e-5)
+    assert float(pair[1]) == pytest.approx(h_ww, rel=1e-5)
+    lx, lv, lw = lst(x), lst(v), lst(w)
+    assert float(pair[0]) == pytest.approx(cas.hessian(lx, lv, lw, d), rel=1e-4)
+    assert float(pair[1]) == pytest.approx(cas.hessian(lx, lw, lw, d), rel=1e-4)
