@@ -37,6 +37,16 @@ struct TomoArgs {
   const int* rp = nullptr;         // [4 n^2 + 1]
   const int* ci = nullptr;         // [nnz] sample index s
   const float* cw = nullptr;       // [nnz] weight wy wx
+  // the samples binned by grid tile for the forward gather: work item i covers
+  // samples [it.z, it.z + it.w) of the sorted list, all in tile (it.x, it.y);
+  // per sorted sample: (local tap origin oy << 16 | ox, sample index) and the
+  // fractional parts (frac(2n x0), frac(2n y0)) that give its weights
+  const int4* items = nullptr;
+  const int2* so = nullptr;
+  const float2* sf = nullptr;
+  int ts = 0;                      // tile edge on the 2n grid
+  int nitems = 0;
+  float cxs = 0.f, cys = 0.f;      // weights exp(cxs (i - m - f)^2), cys exp(...)
   float2* W = nullptr;             // [nz, n, 2n]
   float2* G = nullptr;             // [nz, 2n, 2n]
   float2* Sh = nullptr;            // [ntheta n, nz] transformed sinogram (RT)
@@ -161,6 +171,91 @@ __global__ __launch_bounds__(256) void tomo_gather_k(TomoArgs a, TO* __restrict_
   fft_lds<N, E, true>(s, t, twr);
   if (valid) {
     TO* dst = out + ((int64_t)th * a.nz + z) * N;
+    for (int e = 0; e < E; ++e) {
+      const int tt = t + e * T;
+      st_c<TO>(dst, tt, cscale(s[fpad(tt)], -sgnf(tt) / (4.0f * N)));
+    }
+  }
+}
+
+// ---- forward pass C, tiled: gather through LDS tiles of the grid ------------
+// The polar samples are binned by the grid tile holding the first tap of
+// their 9x9 window (plan time).  A workgroup takes one work item (up to 256
+// samples of one tile) for kTZ slices: it stages the tile plus the window
+// margin, wrapped on the 2n grid, into LDS for each slice, and each thread
+// gathers its sample's taps from LDS.  The weights come from the sample's
+// fractional grid offset, exp(-pi^2/mu ((i - m - f)/(2n))^2) evaluated in
+// FP32 (the FP64-built offset keeps the argument accurate).  The sums run in
+// the order of the untiled gather (rows outer, columns inner).  Output: the
+// line spectra Sh[(theta nz + z) n + t], scaled by c1d / n.
+constexpr int kTS = 32, kTL = 9, kTP = kTS + kTL - 1, kTZ = 4;
+__global__ __launch_bounds__(256) void tomo_tile_gather_k(TomoArgs a) {
+  __shared__ float2 tile[kTZ * kTP * kTP];
+  const int4 it = a.items[blockIdx.x];
+  const int z0 = blockIdx.y * kTZ;
+  const int n2 = 2 * a.n, P = a.ts + kTL - 1;
+  const int64_t plane = (int64_t)n2 * n2;
+  const int gy0 = it.y * a.ts, gx0 = it.x * a.ts;
+  for (int idx = threadIdx.x; idx < kTZ * P * P; idx += 256) {
+    const int q = idx / (P * P), rem = idx % (P * P);
+    const int r = rem / P, c = rem % P;
+    const int z = z0 + q;
+    const int gr = (gy0 + r) % n2, gc = (gx0 + c) % n2;
+    tile[q * kTP * kTP + r * kTP + c] = z < a.nz ? a.G[z * plane + (int64_t)gr * n2 + gc] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  if ((int)threadIdx.x >= it.w) return;
+  const int k = it.z + threadIdx.x;
+  const int2 so = a.so[k];
+  const float2 f = a.sf[k];
+  const int ox = so.x & 0xffff, oy = so.x >> 16, si = so.y;
+  float wx[kTL], wy[kTL];
+#pragma unroll
+  for (int i = 0; i < kTL; ++i) {
+    const float dx = (float)(i - a.m) - f.x, dy = (float)(i - a.m) - f.y;
+    wx[i] = expf(a.cxs * dx * dx);
+    wy[i] = a.cys * expf(a.cxs * dy * dy);
+  }
+  const int th = si / a.n, t = si % a.n;
+  const float sc = -sgnf(t) / (float)a.n;
+  for (int q = 0; q < kTZ; ++q) {
+    const int z = z0 + q;
+    if (z >= a.nz) break;
+    const float2* tq = tile + q * kTP * kTP + oy * kTP + ox;
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int i1 = 0; i1 < kTL; ++i1) {
+      float2 racc = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int i0 = 0; i0 < kTL; ++i0) racc = vfma(wx[i0], tq[i1 * kTP + i0], racc);
+      acc = vfma(wy[i1], racc, acc);
+    }
+    a.Sh[((int64_t)th * a.nz + z) * a.n + t] = cscale(acc, sc);
+  }
+}
+
+// ---- forward pass D: line IFFT of the gathered spectra ----------------------
+template <int N, int E, typename TO>
+__global__ __launch_bounds__(256) void tomo_line_ifft_k(TomoArgs a, TO* __restrict__ out) {
+  constexpr int T = N / E, ROWS = 256 / T;
+  __shared__ float2 lds[ROWS * fft_buf<N>()];
+  const int r = threadIdx.x / T, t = threadIdx.x % T;
+  const int64_t line = (int64_t)blockIdx.x * ROWS + r;  // theta nz + z
+  const bool valid = line < (int64_t)a.ntheta * a.nz;
+  float2* s = lds + r * fft_buf<N>();
+  FftTw<N, E> twr;
+  fft_load_twiddles<N, E>(twr, t, a.tw1);
+  const float2* src = a.Sh + (valid ? line : 0) * N;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int tt = t + e * T;
+    s[fpad(tt)] = src[tt];
+  }
+  __syncthreads();
+  fft_lds<N, E, true>(s, t, twr);
+  if (valid) {
+    TO* dst = out + line * N;
+#pragma unroll
     for (int e = 0; e < E; ++e) {
       const int tt = t + e * T;
       st_c<TO>(dst, tt, cscale(s[fpad(tt)], -sgnf(tt) / (4.0f * N)));
@@ -326,7 +421,16 @@ static hipError_t tomo_fwd_n(const TomoArgs& a, const TIO* obj, TIO* out, hipStr
   constexpr int EL = N >= 8 ? 8 : N;
   constexpr int RL = 256 / (N / EL);
   const int64_t lines = (int64_t)a.ntheta * a.nz;
-  hipLaunchKernelGGL((tomo_gather_k<N, EL, TIO>), dim3((unsigned)((lines + RL - 1) / RL)), dim3(256), 0, st, a, out);
+  if (a.items) {
+    hipLaunchKernelGGL(tomo_tile_gather_k, dim3((unsigned)a.nitems, (unsigned)((a.nz + kTZ - 1) / kTZ)), dim3(256),
+                       0, st, a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL((tomo_line_ifft_k<N, EL, TIO>), dim3((unsigned)((lines + RL - 1) / RL)), dim3(256), 0, st, a,
+                       out);
+  } else {
+    hipLaunchKernelGGL((tomo_gather_k<N, EL, TIO>), dim3((unsigned)((lines + RL - 1) / RL)), dim3(256), 0, st, a,
+                       out);
+  }
   return hipGetLastError();
 }
 
@@ -387,6 +491,10 @@ struct htamd_tomo_plan {
   int* rp = nullptr;
   int* ci = nullptr;
   float* cw = nullptr;
+  int4* items = nullptr;
+  int2* so = nullptr;
+  float2* sf = nullptr;
+  int nitems = 0, ts = 0;
 };
 
 extern "C" {
@@ -463,6 +571,35 @@ int htamd_tomo_plan_create(int64_t n, int64_t ntheta, const double* theta, const
         cw[(size_t)k] = wy * tw[(size_t)(i0 * S + si)];
       }
     }
+  // Tile binning of the samples for the forward gather: tile of the window's
+  // first tap (wrapped), sorted by tile (counting sort, samples in increasing
+  // order within a tile), cut into work items of at most 256 samples.
+  const int ts = (int)std::min<int64_t>(32, n2);
+  const int ntx = (int)(n2 / ts);
+  std::vector<int> tcount((size_t)(ntx * ntx + 1), 0);
+  for (int64_t si = 0; si < S; ++si)
+    ++tcount[(size_t)((wrap(tb[(size_t)si].y) / ts) * ntx + wrap(tb[(size_t)si].x) / ts + 1)];
+  for (int i = 0; i < ntx * ntx; ++i) tcount[(size_t)(i + 1)] += tcount[(size_t)i];
+  std::vector<int> tfill(tcount.begin(), tcount.end() - 1);
+  std::vector<int2> so((size_t)S);
+  std::vector<float2> sf((size_t)S);
+  for (int64_t si = 0; si < S; ++si) {
+    const int wx0 = wrap(tb[(size_t)si].x), wy0 = wrap(tb[(size_t)si].y);
+    const int tile = (wy0 / ts) * ntx + wx0 / ts;
+    const int k = tfill[(size_t)tile]++;
+    so[(size_t)k] = make_int2(((wy0 % ts) << 16) | (wx0 % ts), (int)si);
+    const int64_t kk = si / n, t = si % n;
+    const double th = (double)(float)theta[kk];
+    const double x2 = (double)n2 * ((double)(t - n / 2) / (double)n * std::cos(th));
+    const double y2 = (double)n2 * (-(double)(t - n / 2) / (double)n * std::sin(th));
+    sf[(size_t)k] = make_float2((float)(x2 - std::floor(x2)), (float)(y2 - std::floor(y2)));
+  }
+  std::vector<int4> items;
+  for (int tile = 0; tile < ntx * ntx; ++tile)
+    for (int b = tcount[(size_t)tile]; b < tcount[(size_t)(tile + 1)]; b += 256)
+      items.push_back(make_int4(tile % ntx, tile / ntx, b, std::min(256, tcount[(size_t)(tile + 1)] - b)));
+  p->nitems = (int)items.size();
+  p->ts = ts;
   // deapodiser phi (tomo.py:19-21, 46-47) with the mask folded in
   std::vector<float> phi((size_t)(n * n));
   for (int64_t i = 0; i < n; ++i) {
@@ -490,7 +627,8 @@ int htamd_tomo_plan_create(int64_t n, int64_t ntheta, const double* theta, const
   if (e != hipSuccess || (e = up(&p->phi, phi)) != hipSuccess || (e = up(&p->tw2, twid(2 * n))) != hipSuccess ||
       (e = up(&p->tw1, twid(n))) != hipSuccess || (e = up(&p->tb, tb)) != hipSuccess ||
       (e = up(&p->tw, tw)) != hipSuccess || (e = up(&p->rp, rp)) != hipSuccess || (e = up(&p->ci, ci)) != hipSuccess ||
-      (e = up(&p->cw, cw)) != hipSuccess) {
+      (e = up(&p->cw, cw)) != hipSuccess || (e = up(&p->items, items)) != hipSuccess ||
+      (e = up(&p->so, so)) != hipSuccess || (e = up(&p->sf, sf)) != hipSuccess) {
     htamd_tomo_plan_destroy(p);
     return hip_fail(e, "tomo_plan_create");
   }
@@ -508,6 +646,9 @@ int htamd_tomo_plan_destroy(htamd_tomo_plan* p) {
   (void)hipFree(p->rp);
   (void)hipFree(p->ci);
   (void)hipFree(p->cw);
+  (void)hipFree(p->items);
+  (void)hipFree(p->so);
+  (void)hipFree(p->sf);
   delete p;
   return HTAMD_OK;
 }
@@ -544,6 +685,13 @@ static int tomo_run(const htamd_tomo_plan* p, bool adjoint, const void* in, int 
   a.cw = p->cw;
   a.tw2 = p->tw2;
   a.tw1 = p->tw1;
+  a.items = p->m == (kTL - 1) / 2 ? p->items : nullptr;  // the tiled gather is built for 9x9 windows
+  a.so = p->so;
+  a.sf = p->sf;
+  a.ts = p->ts;
+  a.nitems = p->nitems;
+  a.cxs = (float)(-M_PI * M_PI / p->mu / (4.0 * (double)p->n * (double)p->n));
+  a.cys = (float)(M_PI / p->mu);
   a.W = (float2*)ws;
   a.G = (float2*)ws + nz * p->n * 2 * p->n;
   a.Sh = a.G + nz * 4 * p->n * p->n;
