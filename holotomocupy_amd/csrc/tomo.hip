@@ -15,6 +15,7 @@
 // a slice-contiguous copy of the transformed sinogram -- the reference's atomic
 // scatter in a fixed summation order.
 #include <cmath>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -35,8 +36,7 @@ struct TomoArgs {
   const float* tw = nullptr;       // [2L][ntheta n]: wx[i] at tw[i S + s], wy[i] at tw[(L + i) S + s]
   // the same taps transposed, one row per grid point g = row 2n + col (CSR):
   const int* rp = nullptr;         // [4 n^2 + 1]
-  const int* ci = nullptr;         // [nnz] sample index s
-  const float* cw = nullptr;       // [nnz] weight wy wx
+  const int2* csw = nullptr;       // [nnz] (sample index s, weight wy wx as float bits)
   // the samples binned by grid tile for the forward gather: work item i covers
   // samples [it.z, it.z + it.w) of the sorted list, all in tile (it.x, it.y);
   // per sorted sample: (local tap origin oy << 16 | ox, sample index) and the
@@ -292,38 +292,40 @@ __global__ __launch_bounds__(256) void tomo_linefft_k(TomoArgs a, const TI* __re
 }
 
 // ---- adjoint pass C'': gather onto the grid (transpose of the forward taps) ---
-// Thread per grid point and kZB consecutive slices; the contributions of a
-// grid point are summed in sample order (deterministic, no atomics).
-constexpr int kZB = 16;
-template <bool FULL>
+// Thread per grid point and ZB consecutive slices; the contributions of a
+// grid point are summed in sample order (deterministic, no atomics).  Each
+// CSR entry is one 8-byte (sample, weight) pair, and ZB = 32 where the slice
+// count allows, so the index and weight loads are amortised over more slices.
+template <int ZB, bool FULL>
 __global__ __launch_bounds__(256) void tomo_grid_gather_k(TomoArgs a) {
   const int64_t n4 = 4LL * a.n * a.n;
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n4) return;
-  const int z0 = blockIdx.y * kZB;
-  float2 acc[kZB];
+  const int z0 = blockIdx.y * ZB;
+  float2 acc[ZB];
 #pragma unroll
-  for (int q = 0; q < kZB; ++q) acc[q] = make_float2(0.f, 0.f);
+  for (int q = 0; q < ZB; ++q) acc[q] = make_float2(0.f, 0.f);
   const int k0 = a.rp[g], k1 = a.rp[g + 1];
   for (int k = k0; k < k1; ++k) {
-    const float w = a.cw[k];
-    const float2* src = a.Sh + (int64_t)a.ci[k] * a.nz + z0;
-    if (FULL) {  // kZB whole slices, 16-byte aligned (nz % kZB == 0): pairs as float4
+    const int2 cw = a.csw[k];
+    const float w = __int_as_float(cw.y);
+    const float2* src = a.Sh + (int64_t)cw.x * a.nz + z0;
+    if (FULL) {  // ZB whole slices, 16-byte aligned (nz % ZB == 0): pairs as float4
       const float4* s4 = (const float4*)src;
 #pragma unroll
-      for (int q = 0; q < kZB / 2; ++q) {
+      for (int q = 0; q < ZB / 2; ++q) {
         const float4 v = s4[q];
         acc[2 * q] = vfma(w, make_float2(v.x, v.y), acc[2 * q]);
         acc[2 * q + 1] = vfma(w, make_float2(v.z, v.w), acc[2 * q + 1]);
       }
     } else {
 #pragma unroll
-      for (int q = 0; q < kZB; ++q)
+      for (int q = 0; q < ZB; ++q)
         if (z0 + q < a.nz) acc[q] = vfma(w, src[q], acc[q]);
     }
   }
 #pragma unroll
-  for (int q = 0; q < kZB; ++q)
+  for (int q = 0; q < ZB; ++q)
     if (FULL || z0 + q < a.nz) a.G[(int64_t)(z0 + q) * n4 + g] = acc[q];
 }
 
@@ -446,11 +448,13 @@ static hipError_t tomo_adj_n(const TomoArgs& a, const TIO* data, TIO* out, hipSt
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int64_t n4 = (int64_t)N2 * N2;
-  const dim3 gg((unsigned)((n4 + 255) / 256), (unsigned)((a.nz + kZB - 1) / kZB));
-  if (a.nz % kZB == 0)
-    hipLaunchKernelGGL(tomo_grid_gather_k<true>, gg, dim3(256), 0, st, a);
+  const unsigned gx = (unsigned)((n4 + 255) / 256);
+  if (a.nz % 32 == 0)
+    hipLaunchKernelGGL((tomo_grid_gather_k<32, true>), dim3(gx, (unsigned)(a.nz / 32)), dim3(256), 0, st, a);
+  else if (a.nz % 16 == 0)
+    hipLaunchKernelGGL((tomo_grid_gather_k<16, true>), dim3(gx, (unsigned)(a.nz / 16)), dim3(256), 0, st, a);
   else
-    hipLaunchKernelGGL(tomo_grid_gather_k<false>, gg, dim3(256), 0, st, a);
+    hipLaunchKernelGGL((tomo_grid_gather_k<16, false>), dim3(gx, (unsigned)((a.nz + 15) / 16)), dim3(256), 0, st, a);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL((tomo_cols_adj_k<N2, E, C>), dim3((unsigned)(a.nz * (N2 / C))), dim3(C * (N2 / E)), 0, st, a);
   if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -489,8 +493,7 @@ struct htamd_tomo_plan {
   int2* tb = nullptr;
   float* tw = nullptr;
   int* rp = nullptr;
-  int* ci = nullptr;
-  float* cw = nullptr;
+  int2* csw = nullptr;
   int4* items = nullptr;
   int2* so = nullptr;
   float2* sf = nullptr;
@@ -559,16 +562,17 @@ int htamd_tomo_plan_create(int64_t n, int64_t ntheta, const double* theta, const
         ++rp[(size_t)(wrap(tb[(size_t)si].y + i1) * n2 + wrap(tb[(size_t)si].x + i0) + 1)];
   for (int64_t g = 0; g < ng; ++g) rp[(size_t)(g + 1)] += rp[(size_t)g];
   std::vector<int> fill(rp.begin(), rp.end() - 1);
-  std::vector<int> ci((size_t)(S * L * L));
-  std::vector<float> cw((size_t)(S * L * L));
+  std::vector<int2> csw((size_t)(S * L * L));
   for (int64_t si = 0; si < S; ++si)
     for (int i1 = 0; i1 < L; ++i1) {
       const float wy = tw[(size_t)((L + i1) * S + si)];
       for (int i0 = 0; i0 < L; ++i0) {
         const int64_t g = wrap(tb[(size_t)si].y + i1) * n2 + wrap(tb[(size_t)si].x + i0);
         const int k = fill[(size_t)g]++;
-        ci[(size_t)k] = (int)si;
-        cw[(size_t)k] = wy * tw[(size_t)(i0 * S + si)];
+        const float w = wy * tw[(size_t)(i0 * S + si)];
+        int wb;
+        std::memcpy(&wb, &w, sizeof wb);
+        csw[(size_t)k] = make_int2((int)si, wb);
       }
     }
   // Tile binning of the samples for the forward gather: tile of the window's
@@ -626,8 +630,8 @@ int htamd_tomo_plan_create(int64_t n, int64_t ntheta, const double* theta, const
   };
   if (e != hipSuccess || (e = up(&p->phi, phi)) != hipSuccess || (e = up(&p->tw2, twid(2 * n))) != hipSuccess ||
       (e = up(&p->tw1, twid(n))) != hipSuccess || (e = up(&p->tb, tb)) != hipSuccess ||
-      (e = up(&p->tw, tw)) != hipSuccess || (e = up(&p->rp, rp)) != hipSuccess || (e = up(&p->ci, ci)) != hipSuccess ||
-      (e = up(&p->cw, cw)) != hipSuccess || (e = up(&p->items, items)) != hipSuccess ||
+      (e = up(&p->tw, tw)) != hipSuccess || (e = up(&p->rp, rp)) != hipSuccess || (e = up(&p->csw, csw)) != hipSuccess ||
+      (e = up(&p->items, items)) != hipSuccess ||
       (e = up(&p->so, so)) != hipSuccess || (e = up(&p->sf, sf)) != hipSuccess) {
     htamd_tomo_plan_destroy(p);
     return hip_fail(e, "tomo_plan_create");
@@ -644,8 +648,7 @@ int htamd_tomo_plan_destroy(htamd_tomo_plan* p) {
   (void)hipFree(p->tb);
   (void)hipFree(p->tw);
   (void)hipFree(p->rp);
-  (void)hipFree(p->ci);
-  (void)hipFree(p->cw);
+  (void)hipFree(p->csw);
   (void)hipFree(p->items);
   (void)hipFree(p->so);
   (void)hipFree(p->sf);
@@ -681,8 +684,7 @@ static int tomo_run(const htamd_tomo_plan* p, bool adjoint, const void* in, int 
   a.tb = p->tb;
   a.tw = p->tw;
   a.rp = p->rp;
-  a.ci = p->ci;
-  a.cw = p->cw;
+  a.csw = p->csw;
   a.tw2 = p->tw2;
   a.tw1 = p->tw1;
   a.items = p->m == (kTL - 1) / 2 ? p->items : nullptr;  // the tiled gather is built for 9x9 windows
