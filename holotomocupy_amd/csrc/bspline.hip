@@ -226,7 +226,7 @@ __global__ __launch_bounds__(256) void bs_dadj_final_k(const float2* __restrict_
 // FP64; the dropped tail is below 1e-8 relative).  One pass: a (TY+2H) x
 // (TX+2H) tile with periodic halo is staged in LDS, filtered along x into a
 // second LDS tile, then along y, and written once.
-constexpr int kPfTX = 64, kPfTY = 32;
+constexpr int kPfTX = 64, kPfTY = 32, kPfR = 8;
 
 template <typename T>
 __global__ __launch_bounds__(256) void bs_prefilter_k(const T* __restrict__ in, T* __restrict__ out,
@@ -253,22 +253,39 @@ __global__ __launch_bounds__(256) void bs_prefilter_k(const T* __restrict__ in, 
     s_in[ly][lx] = src[(int64_t)gy * npsi + gx];
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < LY * kPfTX; i += 256) {
-    const int ly = i / kPfTX, lx = i % kPfTX;
-    T acc = vzero<T>();
+  // each thread filters kPfR consecutive outputs from a register window of
+  // kPfR + 2H inputs (one LDS read per input instead of one per tap);
+  // along x consecutive lanes take consecutive rows (distinct LDS banks)
+  constexpr int R = kPfR, WIN = kPfR + 2 * H;
+  for (int i = threadIdx.x; i < LY * (kPfTX / R); i += 256) {
+    const int ly = i % LY, gx = (i / LY) * R;
+    T win[WIN];
 #pragma unroll
-    for (int d = 0; d <= 2 * H; ++d) acc = vfma(taps.wx[d], s_in[ly][lx + d], acc);
-    s_tmp[ly][lx] = acc;
+    for (int k = 0; k < WIN; ++k) win[k] = s_in[ly][gx + k];
+#pragma unroll
+    for (int o = 0; o < R; ++o) {
+      T acc = vzero<T>();
+#pragma unroll
+      for (int d = 0; d <= 2 * H; ++d) acc = vfma(taps.wx[d], win[o + d], acc);
+      s_tmp[ly][gx + o] = acc;
+    }
   }
   __syncthreads();
   T* dst = out + b * plane;
-  for (int i = threadIdx.x; i < kPfTY * kPfTX; i += 256) {
-    const int ly = i / kPfTX, lx = i % kPfTX;
-    if (y0 + ly >= nzpsi || x0 + lx >= npsi) continue;
-    T acc = vzero<T>();
+  for (int i = threadIdx.x; i < kPfTX * (kPfTY / R); i += 256) {
+    const int lx = i % kPfTX, gy = (i / kPfTX) * R;
+    if (x0 + lx >= npsi) continue;
+    T win[WIN];
 #pragma unroll
-    for (int d = 0; d <= 2 * H; ++d) acc = vfma(taps.wy[d], s_tmp[ly + d][lx], acc);
-    dst[(int64_t)(y0 + ly) * npsi + x0 + lx] = acc;
+    for (int k = 0; k < WIN; ++k) win[k] = s_tmp[gy + k][lx];
+#pragma unroll
+    for (int o = 0; o < R; ++o) {
+      if (y0 + gy + o >= nzpsi) break;
+      T acc = vzero<T>();
+#pragma unroll
+      for (int d = 0; d <= 2 * H; ++d) acc = vfma(taps.wy[d], win[o + d], acc);
+      dst[(int64_t)(y0 + gy + o) * npsi + x0 + lx] = acc;
+    }
   }
 }
 
