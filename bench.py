@@ -36,10 +36,11 @@ def parse():
     p.add_argument("--path", choices=["auto", "plain", "fused", "served"], default="auto")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison run")
-    p.add_argument("--op", choices=["prop", "tomo", "shift", "bh"], default=None,
+    p.add_argument("--op", choices=["prop", "tomo", "shift", "bh", "serve-mix"], default=None,
                    help="operator benchmark instead of the BH gradient: prop = D then DT, 16 fields of "
                         "256^2 at 4 distances; tomo = R and RT at 256^3, 384 angles; shift = coeff, S and "
-                        "Sadj at 384 x 256^2; bh = Rec.BH iterations at 512^3, 720 angles, 1024^2")
+                        "Sadj at 384 x 256^2; bh = Rec.BH iterations at 512^3, 720 angles, 1024^2; serve-mix = "
+                        "Poisson mix of operator, gradient and bilinear-form requests (n 256..2048)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="first run a 256 MiB device-to-device copy 3 times (a known-size kernel for "
                         "scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -410,6 +411,141 @@ def op_bh(a):
     return line
 
 
+def op_serve_mix(a, nreq=40, rate=200.0, seed=1005):
+    """Serving mix: Poisson arrivals at `rate` requests/s of operator (D or DT of
+    A fields at one distance), BH gradient and BH bilinear-form requests
+    (weights 0.4 / 0.4 / 0.2), detector and object grid n in {256, 512, 1024, 2048}
+    (weights 0.4 / 0.3 / 0.2 / 0.1, object n x n), 1-4 distances, 16-128
+    angles.  Served: BHServer (admission by HBM budget, batches per bucket, HIP
+    graphs for gradient batches).  Plain: the same requests one at a time through
+    the per-operator path (gradients and bilinear forms over 16-angle chunks)."""
+    import threading
+    import numpy as np
+    import torch
+    from holotomocupy_amd import phantom
+    from holotomocupy_amd.rec import Rec
+    from holotomocupy_amd.serve import BHServer, BilinearRequest, Geometry, GradientRequest, OperatorRequest
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    sizes, kinds = [256, 512, 1024, 2048], ["op", "grad", "hess"]
+    specs = []
+    for _ in range(nreq):
+        n = int(rng.choice(sizes, p=[0.4, 0.3, 0.2, 0.1]))
+        specs.append((str(rng.choice(kinds, p=[0.4, 0.4, 0.2])), n, int(rng.integers(1, 5)),
+                      int(rng.integers(16, 129))))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    geos, probes = {}, {}
+
+    def geo(n, nd):
+        if (n, nd) not in geos:
+            g = geometry(n, nd)
+            geos[(n, nd)] = Geometry(n, n, n, n, nd, energy=g["energy"], z1=tuple(float(z) for z in g["z1"]),
+                                     focustodetectordistance=g["focustodetectordistance"],
+                                     detector_pixelsize=g["detector_pixelsize"])
+            probes[(n, nd)] = torch.from_numpy(phantom.probe(rng, nd, n, n)).cuda()
+        return geos[(n, nd)]
+
+    def make(kind, n, nd, A):
+        G = geo(n, nd)
+        if kind == "op":
+            x = torch.randn((A, n, n), dtype=torch.complex64, device="cuda", generator=gen)
+            return OperatorRequest(G, "D" if rng.random() < 0.5 else "DT", int(rng.integers(0, nd)), x)
+        proj = 0.3 * torch.randn((A, n, n), dtype=torch.complex64, device="cuda", generator=gen)
+        pos = 2 * torch.rand((A, nd, 2), device="cuda", generator=gen) - 1
+        data = 0.5 + torch.rand((A, nd, n, n), device="cuda", generator=gen)
+        if kind == "grad":
+            return GradientRequest(G, probes[(n, nd)], proj, pos, data, data_size=float(A * nd * n * n),
+                                   rho=(1.0, 0.05, 0.02))
+        x = {"prb": probes[(n, nd)], "proj": proj, "pos": pos}
+        y = {"prb": 0.1 * probes[(n, nd)], "proj": 0.05 * torch.randn_like(proj), "pos": 0.3 * torch.randn_like(pos)}
+        return BilinearRequest(G, x, y, None, data, float(A * nd * n * n))
+
+    reqs = [make(*sp) for sp in specs]
+    torch.cuda.synchronize()
+    clone = lambda r: (OperatorRequest(r.geometry, r.op, r.j, r.x) if isinstance(r, OperatorRequest) else  # noqa: E731
+                       GradientRequest(r.geometry, r.prb, r.proj, r.pos, r.data, r.data_size, r.rho)
+                       if isinstance(r, GradientRequest) else
+                       BilinearRequest(r.geometry, r.x, r.y, r.z, r.data, r.data_size))
+    srv = BHServer(max_batch_angles=128)
+
+    def serve_pass():
+        timed = [clone(r) for r in reqs]
+        t0 = time.perf_counter()
+
+        def feeder():
+            for r, t in zip(timed, arrivals):
+                d = t0 + t - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                srv.submit(r)
+
+        th = threading.Thread(target=feeder)
+        th.start()
+        th.join()
+        for r in timed:
+            r.future.result()
+        return timed, time.perf_counter() - t0
+
+    serve_pass()  # rehearsal with the same arrivals: plans, staging, graphs of the batch shapes
+    torch.cuda.synchronize()
+    timed, wall_s = serve_pass()
+    srv.close()
+    lat = sorted((r.t_done - r.t_submit) * 1e3 for r in timed)
+    # plain: one request at a time, per-operator kernels, 16-angle chunks
+    recs = {}
+
+    def rec_for(G):
+        if G not in recs:
+            g = geometry(G.n, G.ndist)
+            from types import SimpleNamespace
+            recs[G] = Rec(SimpleNamespace(ndist=G.ndist, ntheta=16, nz=G.n, n=G.n, nzobj=G.n, nobj=G.n,
+                                          obj_dtype="complex64", rho=[1.0, 0.05, 0.02], nchunk=16, **g))
+        return recs[G]
+
+    def plain(r):
+        rec = rec_for(r.geometry)
+        if isinstance(r, OperatorRequest):
+            return (rec.cl_prop.D if r.op == "D" else rec.cl_prop.DT)(r.x, r.j)
+        A = r.nangles
+        if isinstance(r, GradientRequest):
+            rec.data_size = r.data_size
+            gr = {"prb": torch.zeros_like(r.prb)}
+            for st in range(0, A, 16):
+                B = min(16, A - st)
+                sl = {"prb": r.prb, "proj": r.proj[st:st + B], "pos": r.pos[st:st + B]}
+                g = {"prb": gr["prb"], "proj": torch.empty_like(sl["proj"]), "pos": torch.empty_like(sl["pos"])}
+                rec._chunk(0, B)
+                rec.gradients_cascade(sl, g, r.data[st:st + B])
+            return gr
+        rec.data_size = r.data_size
+        tot = 0.0
+        for st in range(0, A, 16):
+            B = min(16, A - st)
+            x = {"prb": r.x["prb"], "proj": r.x["proj"][st:st + B], "pos": r.x["pos"][st:st + B]}
+            y = {"prb": r.y["prb"], "proj": r.y["proj"][st:st + B], "pos": r.y["pos"][st:st + B]}
+            tot += rec.hessian_cascade(x, y, y, r.data[st:st + B])
+        return tot
+
+    for r in reqs[:4]:
+        plain(r)
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    for r in reqs:
+        plain(r)
+        torch.cuda.synchronize()
+    plain_s = time.perf_counter() - tp
+    counts = {k: sum(1 for sp in specs if sp[0] == k) for k in kinds}
+    return {"metric": "Serving mix requests/s (served vs plain), p50/p99 latency", "value": round(nreq / wall_s, 3),
+            "unit": "requests/s", "n_gpus": 1, "steps": nreq, "warmup": nreq, "ms_per_step": round(wall_s * 1e3 / nreq, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64",
+            "data": "synthetic (seeded)",
+            "config": {"workload": f"{nreq} requests, Poisson arrivals at {rate:g}/s; n in 256..2048, 1-4 distances, "
+                                   f"16-128 angles; kinds operator/gradient/bilinear = {counts}"},
+            "plain_value": round(nreq / plain_s, 3), "speedup_vs_plain": round(plain_s / wall_s, 3),
+            "latency_ms": {"p50": round(lat[len(lat) // 2], 3), "p99": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))], 3)},
+            "wall_s": round(wall_s, 3), "plain_s": round(plain_s, 3)}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -419,7 +555,8 @@ def main():
     import torch
     torch.cuda.set_device(0)
     if a.op:
-        print(json.dumps({"prop": op_prop, "tomo": op_tomo, "shift": op_shift, "bh": op_bh}[a.op](a)), flush=True)
+        print(json.dumps({"prop": op_prop, "tomo": op_tomo, "shift": op_shift, "bh": op_bh,
+                          "serve-mix": op_serve_mix}[a.op](a)), flush=True)
         return
     rec, vars_, data, host = make_inputs(a)
     grads = {"prb": torch.zeros_like(vars_["prb"]), "proj": torch.empty_like(vars_["proj"]),

@@ -72,22 +72,28 @@ class BHGradient:
                   _lib.ptr(objective) if objective is not None else None,
                   _lib.ptr(ws), ws.numel(), _lib.stream_ptr())
 
-    def hessian_workspace(self, nangles, same):
+    def hessian_workspace_bytes(self, nangles, same):
         """same: False (general form), True (z = y) or 2 (the pair form)."""
         nb = ctypes.c_int64()
         mode = 2 if same == 2 and same is not True else (1 if same else 0)
         _lib.call("htamd_bh_hessian_workspace_size", self._plan, int(nangles), mode, ctypes.byref(nb))
-        if getattr(self, "_hws", None) is None or self._hws.numel() < nb.value:
+        return nb.value
+
+    def hessian_workspace(self, nangles, same):
+        """The object's own bilinear-form workspace (grown on demand; calls on
+        different streams must pass their own)."""
+        nb = self.hessian_workspace_bytes(nangles, same)
+        if getattr(self, "_hws", None) is None or self._hws.numel() < nb:
             self._hws = None  # release the old one first
-            self._hws = torch.empty(nb.value, dtype=torch.uint8, device=self.device)
+            self._hws = torch.empty(nb, dtype=torch.uint8, device=self.device)
         return self._hws
 
-    def hessian(self, x, y, z, data, mag, out, accumulate=False):
+    def hessian(self, x, y, z, data, mag, out, accumulate=False, scale=None, workspace=None):
         """Fused bilinear form of the data term (Rec.hessian_cascade,
         rec_mpi.py:250-303) for the angles of x['proj']: out (float64 device
-        tensor [1]) (+)= 2/data_size sum [...] (htamd_bh_hessian).  x, y, z are
-        {'prb', 'proj', 'pos'} dicts of device tensors; z is y means the same
-        direction (three propagations instead of four)."""
+        tensor [1]) (+)= scale sum [...] (htamd_bh_hessian), scale = 2/data_size
+        by default.  x, y, z are {'prb', 'proj', 'pos'} dicts of device tensors;
+        z is y means the same direction (three propagations instead of four)."""
         same = z is y
         B = x["proj"].shape[0]
         parts = [x["proj"], x["pos"], mag, x["prb"], y["proj"], y["pos"], y["prb"]]
@@ -96,12 +102,14 @@ class BHGradient:
         for t in parts + [data, out]:
             if not t.is_contiguous() or t.device != self.device:
                 raise ValueError(f"bh hessian: every operand must be a contiguous tensor on {self.device}")
-        ws = self.hessian_workspace(B, same)
+        ws = self.hessian_workspace(B, same) if workspace is None else workspace
+        if ws.numel() < self.hessian_workspace_bytes(B, same):
+            raise ValueError("bh hessian: workspace too small")
         zz = [None, None, None] if same else [_lib.ptr(z["proj"]), _lib.ptr(z["pos"]), _lib.ptr(z["prb"])]
         _lib.call("htamd_bh_hessian", self._plan, B, _lib.ptr(x["proj"]), _lib.ptr(x["pos"]), _lib.ptr(mag),
                   _lib.ptr(x["prb"]), _lib.ptr(y["proj"]), _lib.ptr(y["pos"]), _lib.ptr(y["prb"]), *zz,
-                  _lib.ptr(data), float(2.0 / self.rec.data_size), _lib.ptr(out), 1 if accumulate else 0,
-                  _lib.ptr(ws), ws.numel(), _lib.stream_ptr())
+                  _lib.ptr(data), float(2.0 / self.rec.data_size if scale is None else scale), _lib.ptr(out),
+                  1 if accumulate else 0, _lib.ptr(ws), ws.numel(), _lib.stream_ptr())
         return out
 
     def hessian_pair(self, x, y, z, data, mag, out, accumulate=False):

@@ -1,7 +1,11 @@
 """Continuous-batching request loop for BH operator solves on one GPU.
 
-Requests (a chunk of angles of some reconstruction: probe, projected object,
-positions, measured amplitudes, geometry) are
+Three kinds of requests: BH gradients (the main kind, below), propagations
+D / DT of batches of fields (packed per geometry, operator and distance into
+one launch) and BH bilinear forms (run by the fused bilinear-form kernel, one
+launch sequence per request).  Gradient requests (a chunk of angles of some
+reconstruction: probe, projected object, positions, measured amplitudes,
+geometry) are
   * bucketed by geometry (detector and object grid, distances, physics), since
     one fused launch needs one propagator and one shift plan;
   * admitted while their device footprint (work field, staging, outputs) fits
@@ -67,9 +71,67 @@ class GradientRequest:
     def __post_init__(self):
         self.nangles = int(self.proj.shape[0])
 
+    @property
+    def key(self):
+        return self.geometry
+
+
+@dataclass
+class OperatorRequest:
+    """D_j or DT_j of a batch of fields (Propagation.D / DT,
+    propagation.py:70-112): x [B, nz, n] complex64 device tensor.  Requests of
+    one geometry, operator and distance share one launch; the result is the
+    request's [B, nz, n] output."""
+    geometry: Geometry
+    op: str
+    j: int
+    x: object
+    nangles: int = 0
+    future: Future = field(default_factory=Future)
+    t_submit: float = 0.0
+    t_done: float = 0.0
+    nbytes: int = 0
+
+    def __post_init__(self):
+        if self.op not in ("D", "DT"):
+            raise ValueError("op must be 'D' or 'DT'")
+        self.nangles = int(self.x.shape[0])
+
+    @property
+    def key(self):
+        return (self.geometry, "prop", self.op, int(self.j))
+
+
+@dataclass
+class BilinearRequest:
+    """The BH bilinear form d^2F(x)(y, z) of the data term over the request's
+    angles (Rec.hessian_cascade, rec_mpi.py:250-303): x, y, z are
+    {'prb', 'proj', 'pos'} dicts of device tensors (z None: z = y), data
+    [A, ndist, nz, n].  The result is a float64 device tensor [1]."""
+    geometry: Geometry
+    x: dict
+    y: dict
+    z: object
+    data: object
+    data_size: float
+    nangles: int = 0
+    future: Future = field(default_factory=Future)
+    t_submit: float = 0.0
+    t_done: float = 0.0
+    nbytes: int = 0
+
+    def __post_init__(self):
+        self.nangles = int(self.x["proj"].shape[0])
+
+    @property
+    def key(self):
+        return (self.geometry, "hess")
+
 
 class Scheduler:
-    """Admission by HBM budget and FIFO batching per geometry bucket.
+    """Admission by HBM budget and FIFO batching per bucket (a geometry for
+    gradients; geometry, operator and distance for propagations; geometry for
+    bilinear forms).
 
     executor.footprint(req) -> bytes the request holds while in flight;
     executor.run(geometry, batch) -> handle (enqueues the batch);
@@ -106,7 +168,7 @@ class Scheduler:
         while self.pending and self.in_use + self.pending[0].nbytes <= self.budget:
             r = self.pending.popleft()
             self.in_use += r.nbytes
-            self.admitted.setdefault(r.geometry, collections.deque()).append(r)
+            self.admitted.setdefault(r.key, collections.deque()).append(r)
 
     def _next_batch(self):
         for geo, q in list(self.admitted.items()):
@@ -184,6 +246,12 @@ class GpuExecutor:
 
     def footprint(self, req):
         g = req.geometry
+        if isinstance(req, OperatorRequest):
+            return 2 * req.nangles * g.nz * g.n * 8  # the packed batch and its output
+        if isinstance(req, BilinearRequest):
+            nf = 4 if req.z is not None else 3
+            osz = 8 if g.obj_dtype == "complex64" else 4
+            return req.nangles * (nf * g.ndist * g.n * g.nz * 8 + 3 * g.nobj * g.nzobj * osz) + (1 << 20)
         A, osz = req.nangles, 8 if g.obj_dtype == "complex64" else 4
         field = g.n * g.nz
         plane = g.nobj * g.nzobj
@@ -229,8 +297,55 @@ class GpuExecutor:
                 st["gpos"][:A], st["gprb"][:P], accumulate_prb=False, probe_index=st["pidx"][:A],
                 scales=st["scales"][:A], rho_prb=st["rho_prb"][:P], workspace=slot["ws"])
 
+    def _run_prop(self, key, batch, s):
+        torch = self.torch
+        geo, _, op, j = key
+        b = self._bucket(geo)
+        with torch.cuda.stream(s):
+            for r in batch:
+                r.x.record_stream(s)
+            x = batch[0].x if len(batch) == 1 else torch.cat([r.x for r in batch])
+            y = b["rec"].cl_prop.DT(x, j) if op == "DT" else b["rec"].cl_prop.D(x, j)
+            outs, o = [], 0
+            for r in batch:
+                outs.append(y[o:o + r.nangles])
+                o += r.nangles
+            ev = torch.cuda.Event()
+            ev.record(s)
+        return ev, outs
+
+    def _run_hess(self, key, batch, s, si):
+        torch = self.torch
+        geo = key[0]
+        b = self._bucket(geo)
+        slot = b["slots"][si]  # each stream has its own workspace
+        outs = []
+        with torch.cuda.stream(s):
+            mag = b["rec"].eff_demagnifications[0]
+            for r in batch:
+                for d in (r.x, r.y) + ((r.z,) if r.z is not None else ()):
+                    for t in d.values():
+                        t.record_stream(s)
+                r.data.record_stream(s)
+                m = mag.expand(r.nangles, -1).contiguous()
+                out = torch.zeros(1, dtype=torch.float64, device=self.device)
+                need = b["bh"].hessian_workspace_bytes(r.nangles, r.z is None)
+                if slot.get("hws") is None or slot["hws"].numel() < need:
+                    slot["hws"] = torch.empty(need, dtype=torch.uint8, device=self.device)
+                b["bh"].hessian(r.x, r.y, r.y if r.z is None else r.z, r.data, m, out, accumulate=True,
+                                scale=2.0 / r.data_size, workspace=slot["hws"])
+                outs.append(out)
+            ev = torch.cuda.Event()
+            ev.record(s)
+        return ev, outs
+
     def run(self, geo, batch):
         torch = self.torch
+        if not isinstance(geo, Geometry):  # operator and bilinear-form buckets
+            si = self.next_stream
+            self.next_stream = (si + 1) % len(self.streams)
+            s = self.streams[si]
+            return self._run_prop(geo, batch, s) if geo[1] == "prop" else self._run_hess(geo, batch, s, si)
         b = self._bucket(geo)
         A = sum(r.nangles for r in batch)
         P = len(batch)
@@ -290,8 +405,10 @@ class GpuExecutor:
 
 
 class BHServer:
-    """Thread-backed request loop: submit() returns a Future of
-    (gprb [ndist, nz, n], gproj [A, nzobj, nobj], gpos [A, ndist, 2])."""
+    """Thread-backed request loop: submit() returns a Future of the request's
+    result: (gprb [ndist, nz, n], gproj [A, nzobj, nobj], gpos [A, ndist, 2])
+    for a GradientRequest, the [B, nz, n] field for an OperatorRequest, a
+    float64 [1] tensor for a BilinearRequest."""
 
     def __init__(self, hbm_budget=None, max_batch_angles=128, use_graphs=True, executor=None):
         if executor is None:

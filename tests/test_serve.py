@@ -138,3 +138,52 @@ def test_result_same_alone_and_in_mix(gpu, graphs):
         for u, v in zip(a, f.result(timeout=60)):
             assert torch.equal(u, v)
     srv.close()
+
+
+def test_mixed_kinds_are_bucketed_by_key():
+    from holotomocupy_amd.serve import OperatorRequest
+    ex = FakeExec(per_angle=1)
+    s = Scheduler(ex, hbm_budget=10 ** 9, max_batch_angles=64, max_inflight=8)
+    s.submit(req(G1, 16))
+    s.submit(OperatorRequest(G1, "D", 0, _T(8, 8, 8)))
+    s.submit(OperatorRequest(G1, "D", 1, _T(4, 8, 8)))
+    s.submit(OperatorRequest(G1, "D", 0, _T(8, 8, 8)))
+    s.submit(OperatorRequest(G1, "DT", 0, _T(2, 8, 8)))
+    s.step()
+    keys = [k for k, _ in ex.runs]
+    assert keys == [G1, (G1, "prop", "D", 0), (G1, "prop", "D", 1), (G1, "prop", "DT", 0)]
+    assert ex.runs[1][1] == [8, 8]  # the two D_0 requests share a launch
+
+
+@pytest.mark.gpu
+def test_operator_and_bilinear_requests(gpu):
+    """Propagation requests packed into one launch equal the direct D / DT
+    calls bit for bit; bilinear-form requests equal Rec.hessian_fused; mixed
+    with a gradient request."""
+    import torch
+    from holotomocupy_amd.serve import BilinearRequest, OperatorRequest
+    print("[case] serve operator and bilinear requests", flush=True)
+    rng = np.random.default_rng(78)
+    ga = Geometry(64, 64, 48, 48, 3, detector_pixelsize=1.476e-6 * 64)
+    srv = BHServer(hbm_budget=8 << 30)
+    g = torch.Generator(device="cuda").manual_seed(78)
+    xs = [torch.randn((b, 64, 64), dtype=torch.complex64, device="cuda", generator=g) for b in (3, 5, 2)]
+    ops = [OperatorRequest(ga, "D", 1, xs[0]), OperatorRequest(ga, "DT", 2, xs[1]), OperatorRequest(ga, "D", 1, xs[2])]
+    greq = _gpu_request(rng, ga, 4, 1e6, (1, 0.5, 0.2), 1.0)
+    r = _gpu_request(rng, ga, 3, 1e6, (1, 1, 1), 1.0)
+    xdir = {"prb": r.prb, "proj": r.proj, "pos": r.pos}
+    ydir = {"prb": 0.1 * r.prb, "proj": 0.05 * torch.randn_like(r.proj), "pos": 0.3 * torch.randn_like(r.pos)}
+    hreqs = [BilinearRequest(ga, xdir, ydir, None, r.data, 1e6), BilinearRequest(ga, xdir, ydir, xdir, r.data, 1e6)]
+    futs = [srv.submit(q) for q in ops] + [srv.submit(greq)] + [srv.submit(q) for q in hreqs]
+    res = [f.result(timeout=60) for f in futs]
+    b = srv.sched.ex.buckets[ga]
+    prop = b["rec"].cl_prop
+    assert torch.equal(res[0], prop.D(xs[0], 1))
+    assert torch.equal(res[1], prop.DT(xs[1], 2))
+    assert torch.equal(res[2], prop.D(xs[2], 1))
+    mag = b["rec"].eff_demagnifications[0].expand(3, -1).contiguous()
+    for q, got in zip(hreqs, res[4:]):
+        ref = torch.zeros(1, dtype=torch.float64, device="cuda")
+        b["bh"].hessian(q.x, q.y, q.y if q.z is None else q.z, q.data, mag, ref, accumulate=True, scale=2.0 / 1e6)
+        assert float(got) == float(ref)
+    srv.close()
