@@ -8,15 +8,17 @@ reconstruction: probe, projected object, positions, measured amplitudes,
 geometry) are
   * bucketed by geometry (detector and object grid, distances, physics), since
     one fused launch needs one propagator and one shift plan;
-  * admitted while their device footprint (work field, staging, outputs) fits
-    the HBM budget, in arrival order within a bucket;
-  * packed, as they are admitted, into the bucket's device-resident staging
-    arrays (angles of every request contiguous, each request with its own probe
-    slot, its own 2/data_size and rho scales), so one fused gradient launch
-    (holotomocupy_amd.bh, htamd_bh_gradient) serves the whole batch;
+  * admitted while their in-flight device footprint (result copies, packed
+    batches) fits the HBM budget, in arrival order within a bucket;
+  * packed into device-resident staging arrays (angles of every request
+    contiguous, each request with its own probe slot, its own 2/data_size and
+    rho scales), so one fused gradient launch (holotomocupy_amd.bh,
+    htamd_bh_gradient) serves the whole batch; staging and workspaces are views
+    of one byte arena per stream, shared by all geometries;
   * launched on one of two streams, so packing and unpacking of one batch
     overlap the kernels of the previous one; the fused launch sequence of a
-    batch shape (angles, probes) is captured once as a HIP graph and replayed.
+    batch shape (geometry, angles, probes) seen before is captured as a HIP
+    graph and replayed.
 A request's result does not depend on the batch it shares: every per-angle
 quantity is computed by the same arithmetic, and probe gradients are summed
 per probe slot in angle order (deterministic kernels, no atomics).
@@ -25,6 +27,7 @@ The scheduling core (`Scheduler`) is plain Python and is exercised on the CPU
 with a fake executor; `GpuExecutor` runs the HIP kernels.
 """
 import collections
+import gc
 import threading
 import time
 from concurrent.futures import Future
@@ -185,17 +188,26 @@ class Scheduler:
             return geo, batch
         return None, None
 
+    def _release(self, batch, error=None):
+        now = time.perf_counter()
+        for r in batch:
+            r.t_done = now
+            self.in_use -= r.nbytes
+            f = r.future  # a client callback may have dropped it already
+            if error is not None and f is not None and not f.done():
+                f.set_exception(error)
+
     def _retire(self):
         keep = []
         for h, geo, batch in self.inflight:
-            if self.ex.done(h):
+            try:
+                if not self.ex.done(h):
+                    keep.append((h, geo, batch))
+                    continue
                 self.ex.finish(h, batch)
-                now = time.perf_counter()
-                for r in batch:
-                    r.t_done = now
-                    self.in_use -= r.nbytes
-            else:
-                keep.append((h, geo, batch))
+                self._release(batch)
+            except Exception as e:  # a failed batch fails its requests, not the loop
+                self._release(batch, e)
         self.inflight = keep
 
     def step(self):
@@ -207,7 +219,10 @@ class Scheduler:
             geo, batch = self._next_batch()
             if not batch:
                 break
-            self.inflight.append((self.ex.run(geo, batch), geo, batch))
+            try:
+                self.inflight.append((self.ex.run(geo, batch), geo, batch))
+            except Exception as e:
+                self._release(batch, e)
         return (len(self.pending), len(self.inflight), self.in_use) != before
 
     def idle(self):
@@ -215,17 +230,28 @@ class Scheduler:
 
 
 class GpuExecutor:
-    """Runs batches with the fused BH gradient; per-geometry device-resident
-    staging and one captured HIP graph per batch shape."""
+    """Runs batches on the GPU: gradients through the fused BH gradient,
+    propagations through one packed D / DT launch, bilinear forms through the
+    fused bilinear-form kernels.
+
+    Each stream owns one byte arena, shared by every geometry: a stream runs
+    its batches in order, so a batch's staging arrays and workspace are views
+    at fixed offsets of its stream's arena, and the arena only grows (to the
+    largest batch seen).  A gradient batch shape (geometry, angles, probes)
+    runs eagerly the first time and is captured as a HIP graph the second
+    time, then replayed; growing an arena drops the graphs of its stream."""
 
     def __init__(self, use_graphs=True, nstreams=2):
         import torch
         self.torch = torch
         self.device = torch.device("cuda", torch.cuda.current_device())
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(nstreams)]
+        self.slots = [{"arena": None, "graphs": {}} for _ in self.streams]
         self.next_stream = 0
         self.buckets = {}
         self.use_graphs = use_graphs
+        self.seen = collections.Counter()  # gradient batch shapes run so far
+        self.arena_growths = 0
 
     # ---- geometry buckets
     def _bucket(self, geo):
@@ -240,63 +266,85 @@ class GpuExecutor:
                                    detector_pixelsize=geo.detector_pixelsize, obj_dtype=geo.obj_dtype,
                                    rho=[1.0, 1.0, 1.0])
             rec = Rec(args)
-            self.buckets[geo] = {"rec": rec, "bh": BHGradient(rec),
-                                 "slots": [{"staging": {}, "graphs": {}, "ws": None} for _ in self.streams]}
+            self.buckets[geo] = {"rec": rec, "bh": BHGradient(rec)}
         return self.buckets[geo]
 
     def footprint(self, req):
+        """Device bytes a request holds while in flight beyond the arenas: the
+        packed batch and output of a propagation, the result copies of a
+        gradient, the scalar of a bilinear form."""
         g = req.geometry
         if isinstance(req, OperatorRequest):
-            return 2 * req.nangles * g.nz * g.n * 8  # the packed batch and its output
+            return 2 * req.nangles * g.nz * g.n * 8
         if isinstance(req, BilinearRequest):
-            nf = 4 if req.z is not None else 3
-            osz = 8 if g.obj_dtype == "complex64" else 4
-            return req.nangles * (nf * g.ndist * g.n * g.nz * 8 + 3 * g.nobj * g.nzobj * osz) + (1 << 20)
-        A, osz = req.nangles, 8 if g.obj_dtype == "complex64" else 4
-        field = g.n * g.nz
-        plane = g.nobj * g.nzobj
-        work = A * g.ndist * field * 8 + 2 * A * plane * osz + A * g.ndist * g.nz * 8
-        staging = A * (plane * osz * 2 + g.ndist * field * 4 + g.ndist * 16 + 16) + 2 * g.ndist * field * 8
-        return work + staging
+            return 1 << 10
+        osz = 8 if g.obj_dtype == "complex64" else 4
+        return req.nangles * (g.nobj * g.nzobj * osz + g.ndist * 8) + g.ndist * g.n * g.nz * 8
 
-    def _staging(self, slot, geo, A, P):
-        """Device arrays of a batch of A angles and P probes (grown on demand),
-        one set per stream so concurrent batches of a geometry do not share them."""
+    # ---- arenas
+    def _alloc(self, make):
+        """make(); if the device is out of memory (results held by clients
+        count against it), return the allocator's free blocks to the device and
+        try once more."""
+        try:
+            return make()
+        except self.torch.OutOfMemoryError:
+            self.torch.cuda.empty_cache()
+            return make()
+
+    def _arena(self, si, nbytes):
+        slot = self.slots[si]
+        if slot["arena"] is None or slot["arena"].numel() < nbytes:
+            if slot["graphs"]:
+                self.streams[si].synchronize()  # no replay of a graph being dropped is still running
+                slot["graphs"].clear()
+            slot["arena"] = None  # freed on this stream: reuse by later work on it is ordered
+            self.arena_growths += 1
+            slot["arena"] = self._alloc(lambda: self.torch.empty(nbytes, dtype=self.torch.uint8,
+                                                                 device=self.device))
+        return slot["arena"]
+
+    def _views(self, arena, parts):
+        """{name: view} of (name, shape, dtype) parts laid out from offset 0,
+        each at a 256-byte boundary."""
+        out, off = {}, 0
+        for name, shape, dt in parts:
+            nb = int(np.prod(shape)) * self.torch.empty((), dtype=dt).element_size()
+            out[name] = arena[off:off + nb].view(dt).view(shape)
+            off += (nb + 255) // 256 * 256
+        return out
+
+    @staticmethod
+    def _span(parts, element_size):
+        return sum((int(np.prod(sh)) * element_size[dt] + 255) // 256 * 256 for _, sh, dt in parts)
+
+    def _staging_parts(self, geo, A, P):
         torch = self.torch
-        st = slot["staging"]
-        if st.get("A", 0) < A or st.get("P", 0) < P:
-            A2, P2 = max(A, st.get("A", 0)), max(P, st.get("P", 0))
-            cdt = torch.complex64 if geo.obj_dtype == "complex64" else torch.float32
-            dev = self.device
-            st.clear()
-            st.update(A=A2, P=P2,
-                      proj=torch.empty((A2, geo.nzobj, geo.nobj), dtype=cdt, device=dev),
-                      gproj=torch.empty((A2, geo.nzobj, geo.nobj), dtype=cdt, device=dev),
-                      pos=torch.empty((A2, geo.ndist, 2), dtype=torch.float32, device=dev),
-                      gpos=torch.empty((A2, geo.ndist, 2), dtype=torch.float32, device=dev),
-                      mag=torch.empty((A2, geo.ndist), dtype=torch.float32, device=dev),
-                      data=torch.empty((A2, geo.ndist, geo.nz, geo.n), dtype=torch.float32, device=dev),
-                      scales=torch.empty((A2, 4), dtype=torch.float32, device=dev),
-                      pidx=torch.empty((A2,), dtype=torch.int32, device=dev),
-                      prb=torch.empty((P2, geo.ndist, geo.nz, geo.n), dtype=torch.complex64, device=dev),
-                      gprb=torch.empty((P2, geo.ndist, geo.nz, geo.n), dtype=torch.complex64, device=dev),
-                      rho_prb=torch.empty((P2,), dtype=torch.float32, device=dev))
-            slot["graphs"].clear()
+        cdt = torch.complex64 if geo.obj_dtype == "complex64" else torch.float32
+        f32, nd = torch.float32, geo.ndist
+        return [("proj", (A, geo.nzobj, geo.nobj), cdt), ("gproj", (A, geo.nzobj, geo.nobj), cdt),
+                ("pos", (A, nd, 2), f32), ("gpos", (A, nd, 2), f32), ("mag", (A, nd), f32),
+                ("data", (A, nd, geo.nz, geo.n), f32), ("scales", (A, 4), f32), ("pidx", (A,), torch.int32),
+                ("prb", (P, nd, geo.nz, geo.n), torch.complex64), ("gprb", (P, nd, geo.nz, geo.n), torch.complex64),
+                ("rho_prb", (P,), f32)]
+
+    def _gradient_arena(self, si, b, geo, A, P):
+        torch = self.torch
+        parts = self._staging_parts(geo, A, P)
+        esz = {torch.complex64: 8, torch.float32: 4, torch.int32: 4}
+        span = self._span(parts, esz)
+        ws = b["bh"].workspace_bytes(A)
+        arena = self._arena(si, span + ws)
+        st = self._views(arena, parts)
+        st["ws"] = arena[span:span + ws]
         return st
 
-    def _workspace(self, b, slot, A):
-        need = b["bh"].workspace_bytes(A)
-        if slot["ws"] is None or slot["ws"].numel() < need:
-            slot["ws"] = self.torch.empty(need, dtype=self.torch.uint8, device=self.device)
-            slot["graphs"].clear()
-        return slot["ws"]
+    def _launch(self, b, st, A, P):
+        b["bh"](st["prb"], st["proj"], st["pos"], st["data"], st["mag"], st["gproj"], st["gpos"], st["gprb"],
+                accumulate_prb=False, probe_index=st["pidx"], scales=st["scales"], rho_prb=st["rho_prb"],
+                workspace=st["ws"])
 
-    def _launch(self, b, slot, A, P):
-        st = slot["staging"]
-        b["bh"](st["prb"][:P], st["proj"][:A], st["pos"][:A], st["data"][:A], st["mag"][:A], st["gproj"][:A],
-                st["gpos"][:A], st["gprb"][:P], accumulate_prb=False, probe_index=st["pidx"][:A],
-                scales=st["scales"][:A], rho_prb=st["rho_prb"][:P], workspace=slot["ws"])
-
+    # ---- batches
     def _run_prop(self, key, batch, s):
         torch = self.torch
         geo, _, op, j = key
@@ -304,8 +352,12 @@ class GpuExecutor:
         with torch.cuda.stream(s):
             for r in batch:
                 r.x.record_stream(s)
-            x = batch[0].x if len(batch) == 1 else torch.cat([r.x for r in batch])
-            y = b["rec"].cl_prop.DT(x, j) if op == "DT" else b["rec"].cl_prop.D(x, j)
+
+            def apply():
+                x = batch[0].x if len(batch) == 1 else torch.cat([r.x for r in batch])
+                return b["rec"].cl_prop.DT(x, j) if op == "DT" else b["rec"].cl_prop.D(x, j)
+
+            y = self._alloc(apply)
             outs, o = [], 0
             for r in batch:
                 outs.append(y[o:o + r.nangles])
@@ -318,7 +370,6 @@ class GpuExecutor:
         torch = self.torch
         geo = key[0]
         b = self._bucket(geo)
-        slot = b["slots"][si]  # each stream has its own workspace
         outs = []
         with torch.cuda.stream(s):
             mag = b["rec"].eff_demagnifications[0]
@@ -330,10 +381,9 @@ class GpuExecutor:
                 m = mag.expand(r.nangles, -1).contiguous()
                 out = torch.zeros(1, dtype=torch.float64, device=self.device)
                 need = b["bh"].hessian_workspace_bytes(r.nangles, r.z is None)
-                if slot.get("hws") is None or slot["hws"].numel() < need:
-                    slot["hws"] = torch.empty(need, dtype=torch.uint8, device=self.device)
+                ws = self._arena(si, need)[:need]
                 b["bh"].hessian(r.x, r.y, r.y if r.z is None else r.z, r.data, m, out, accumulate=True,
-                                scale=2.0 / r.data_size, workspace=slot["hws"])
+                                scale=2.0 / r.data_size, workspace=ws)
                 outs.append(out)
             ev = torch.cuda.Event()
             ev.record(s)
@@ -341,21 +391,17 @@ class GpuExecutor:
 
     def run(self, geo, batch):
         torch = self.torch
+        si = self.next_stream
+        self.next_stream = (si + 1) % len(self.streams)
+        s = self.streams[si]
         if not isinstance(geo, Geometry):  # operator and bilinear-form buckets
-            si = self.next_stream
-            self.next_stream = (si + 1) % len(self.streams)
-            s = self.streams[si]
             return self._run_prop(geo, batch, s) if geo[1] == "prop" else self._run_hess(geo, batch, s, si)
         b = self._bucket(geo)
         A = sum(r.nangles for r in batch)
         P = len(batch)
-        si = self.next_stream
-        s = self.streams[si]
-        self.next_stream = (si + 1) % len(self.streams)
-        slot = b["slots"][si]
+        slot = self.slots[si]
         with torch.cuda.stream(s):
-            st = self._staging(slot, geo, A, P)
-            self._workspace(b, slot, A)
+            st = self._gradient_arena(si, b, geo, A, P)
             mag = b["rec"].eff_demagnifications[0]
             o = 0
             for p, r in enumerate(batch):
@@ -366,24 +412,38 @@ class GpuExecutor:
                 st["pos"][o:o + a].copy_(r.pos, non_blocking=True)
                 st["data"][o:o + a].copy_(r.data, non_blocking=True)
                 st["mag"][o:o + a].copy_(mag.expand(a, -1), non_blocking=True)
-                sc = torch.tensor([2.0 / r.data_size, r.rho[0] ** 2, r.rho[2] ** 2, 0.0], dtype=torch.float32)
-                st["scales"][o:o + a].copy_(sc.expand(a, -1), non_blocking=True)
+                for c, v in enumerate((2.0 / r.data_size, r.rho[0] ** 2, r.rho[2] ** 2, 0.0)):
+                    st["scales"][o:o + a, c].fill_(v)  # device fills: no host copies on the stream
                 st["pidx"][o:o + a].fill_(p)
                 st["prb"][p].copy_(r.prb, non_blocking=True)
                 st["rho_prb"][p].fill_(r.rho[1] ** 2)
                 o += a
-            key = (A, P)
-            if self.use_graphs:
-                g = slot["graphs"].get(key)
-                if g is None:
-                    self._launch(b, slot, A, P)  # eager run first (warm-up)
-                    g = torch.cuda.CUDAGraph()
-                    with torch.cuda.graph(g, stream=s, capture_error_mode="thread_local"):
-                        self._launch(b, slot, A, P)
-                    slot["graphs"][key] = g
+            key = (geo, A, P)
+            g = slot["graphs"].get(key) if self.use_graphs else None
+            if g is not None:
+                g.replay()
+            elif self.use_graphs and self.seen[key] >= 1:
+                # capture on this stream once its staging work has drained (the
+                # torch.cuda.graph context would also synchronize the whole device
+                # and empty the allocator's cache); no Python GC frees meanwhile
+                s.synchronize()
+                g = torch.cuda.CUDAGraph()
+                gc_on = gc.isenabled()
+                gc.disable()
+                try:
+                    g.capture_begin(capture_error_mode="thread_local")
+                    try:
+                        self._launch(b, st, A, P)
+                    finally:
+                        g.capture_end()
+                finally:
+                    if gc_on:
+                        gc.enable()
+                slot["graphs"][key] = g
                 g.replay()
             else:
-                self._launch(b, slot, A, P)
+                self._launch(b, st, A, P)
+            self.seen[key] += 1
             outs = []
             o = 0
             for p, r in enumerate(batch):
@@ -411,12 +471,14 @@ class BHServer:
     float64 [1] tensor for a BilinearRequest."""
 
     def __init__(self, hbm_budget=None, max_batch_angles=128, use_graphs=True, executor=None):
+        """hbm_budget: device bytes that requests in flight may hold beyond the
+        executor's arenas (default a quarter of what is free at start)."""
         if executor is None:
             import torch
             executor = GpuExecutor(use_graphs=use_graphs)
             if hbm_budget is None:
                 free, _ = torch.cuda.mem_get_info()
-                hbm_budget = int(0.6 * free)
+                hbm_budget = free // 4
         self.sched = Scheduler(executor, hbm_budget if hbm_budget is not None else 1 << 40, max_batch_angles)
         self._lock = threading.Condition()
         self._stop = False

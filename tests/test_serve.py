@@ -187,3 +187,57 @@ def test_operator_and_bilinear_requests(gpu):
         b["bh"].hessian(q.x, q.y, q.y if q.z is None else q.z, q.data, mag, ref, accumulate=True, scale=2.0 / 1e6)
         assert float(got) == float(ref)
     srv.close()
+
+
+class _FailingExec(FakeExec):
+    """Raises in run() for batches of one geometry (e.g. device memory
+    exhausted)."""
+    def run(self, geo, batch):
+        if geo == G2:
+            raise MemoryError("device memory exhausted")
+        return super().run(geo, batch)
+
+
+def test_failed_batch_fails_its_requests_and_the_loop_goes_on():
+    ex = _FailingExec(per_angle=1)
+    srv = BHServer(hbm_budget=10 ** 9, executor=ex)
+    bad = srv.submit(req(G2, 8))
+    good = srv.submit(req(G1, 8))
+    with pytest.raises(MemoryError):
+        bad.result(timeout=5)
+    ex.pending_done.update(range(100))
+    assert good.result(timeout=5) is not None
+    srv.close()
+    assert srv.sched.in_use == 0  # the failed request's bytes were released too
+
+
+@pytest.mark.gpu
+def test_arena_growth_and_graph_capture_keep_results(gpu):
+    """Geometries of growing size share each stream's arena (it grows, its
+    graphs are dropped); shapes seen before run from captured graphs.  Every
+    result equals the eager server's bit for bit."""
+    import torch
+    from holotomocupy_amd.serve import GpuExecutor
+    print("[case] serve arena growth and graph capture", flush=True)
+    gb = Geometry(32, 32, 40, 40, 2, detector_pixelsize=1.476e-6 * 128)
+    ga = Geometry(64, 64, 48, 48, 3, detector_pixelsize=1.476e-6 * 64)
+    specs = [(gb, 3, 2e5, (1, 1, 1), 0.7), (ga, 4, 1e6, (1, 0.5, 0.2), 1.0), (gb, 3, 2e5, (1, 1, 1), 0.7),
+             (ga, 4, 1e6, (1, 0.5, 0.2), 1.0), (ga, 4, 1e6, (1, 0.5, 0.2), 1.0), (gb, 3, 2e5, (1, 1, 1), 0.7)]
+    reqs = [_gpu_request(np.random.default_rng(200 + i), *sp) for i, sp in enumerate(specs)]
+
+    def serve(graphs):
+        ex = GpuExecutor(use_graphs=graphs, nstreams=1)
+        srv = BHServer(hbm_budget=1 << 30, executor=ex)
+        out = []
+        for r in reqs:  # one at a time: every batch is one request
+            q = GradientRequest(r.geometry, r.prb, r.proj, r.pos, r.data, r.data_size, r.rho)
+            out.append(srv.submit(q).result(timeout=60))
+        srv.close()
+        return out, ex
+
+    ref, _ = serve(False)
+    got, ex = serve(True)
+    assert ex.arena_growths == 2 and len(ex.slots[0]["graphs"]) == 2
+    for a, b in zip(ref, got):
+        for x, y in zip(a, b):
+            assert torch.equal(x, y)
