@@ -563,15 +563,17 @@ def op_serve_mix(a, nreq=200, rate=200.0, seed=1005):
             tot += rec.hessian_cascade(x, y, y, r.data[st:st + B])
         return tot
 
-    for r in reqs[:4]:
-        plain(r)
-    torch.cuda.synchronize()
-    tp = time.perf_counter()
-    for r in reqs:
-        plain(r)
+    plain_s = None
+    if not a.no_plain:
+        for r in reqs[:4]:
+            plain(r)
         torch.cuda.synchronize()
-    plain_s = time.perf_counter() - tp
-    say(f"plain {plain_s:.3f} s")
+        tp = time.perf_counter()
+        for r in reqs:
+            plain(r)
+            torch.cuda.synchronize()
+        plain_s = time.perf_counter() - tp
+        say(f"plain {plain_s:.3f} s")
     counts = {k: sum(1 for sp in specs if sp[0] == k) for k in kinds}
     return {"metric": "Serving mix requests/s (served vs plain), p50/p99 latency", "value": round(nreq / cap_s, 3),
             "unit": "requests/s", "n_gpus": 1, "steps": nreq, "warmup": 2 * nreq,
@@ -580,11 +582,12 @@ def op_serve_mix(a, nreq=200, rate=200.0, seed=1005):
             "config": {"workload": f"{nreq} requests; n in 256..2048, 1-4 distances, 16-128 angles; kinds "
                                    f"operator/gradient/bilinear = {counts}; value = all submitted at once; "
                                    f"latency at Poisson arrivals of {rate:g}/s"},
-            "plain_value": round(nreq / plain_s, 3), "speedup_vs_plain": round(plain_s / cap_s, 3),
+            "plain_value": round(nreq / plain_s, 3) if plain_s else None,
+            "speedup_vs_plain": round(plain_s / cap_s, 3) if plain_s else None,
             "offered": {"rate": rate, "wall_s": round(wall_s, 3), "requests_per_s": round(nreq / wall_s, 3),
                         "latency_ms": {"p50": round(lat[len(lat) // 2], 3),
                                        "p99": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))], 3)}},
-            "capacity_s": round(cap_s, 3), "plain_s": round(plain_s, 3)}
+            "capacity_s": round(cap_s, 3), "plain_s": round(plain_s, 3) if plain_s else None}
 
 
 def main():

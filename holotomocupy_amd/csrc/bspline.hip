@@ -289,6 +289,91 @@ __global__ __launch_bounds__(256) void bs_prefilter_k(const T* __restrict__ in, 
   }
 }
 
+// Column-strip form for planes at least kPsTX wide.  A block of kPsNT threads
+// covers kPsW = kPsTX + 2H columns (thread t owns periodic column
+// x0 - H + t) and a strip of `sy` output rows.  Each thread slides a register
+// window of R + 2H rows down its column (coalesced row loads, one periodic
+// wrap test per row); per R-row chunk it writes R y-filtered values to LDS,
+// and after one barrier each of kPsTX / 8 x R work items filters 8
+// consecutive outputs of one row from a register window of 8 + 2H LDS values
+// and stores them.  The LDS rows hold one pad slot per 8 values, so the 16
+// lanes of one ds_read_b64 pass hit distinct banks.
+constexpr int kPsTX = 256, kPsW = kPsTX + 2 * kPfH, kPsNT = 320, kPsR = 8;
+constexpr int kPsRS = kPsW + kPsW / 8 + 1;  // padded LDS row, in elements
+static_assert(kPsNT >= kPsW && kPsNT % 64 == 0, "one thread per column of the panel");
+static_assert(kPsR * (kPsTX / 8) <= kPsNT, "one x work item per thread");
+
+template <typename T>
+__global__ __launch_bounds__(kPsNT) void bs_prefilter_strip_k(const T* __restrict__ in, T* __restrict__ out,
+                                                              int nzpsi, int npsi, int tiles_x, int strips,
+                                                              int sy, PrefilterTaps taps) {
+  constexpr int H = kPfH, R = kPsR, WIN = kPsR + 2 * kPfH, G = kPsTX / 8;
+  __shared__ T tmp[R * kPsRS];
+  const int64_t bid = blockIdx.x;
+  const int strip = (int)(bid % strips);
+  const int64_t rest = bid / strips;
+  const int txt = (int)(rest % tiles_x);
+  const int64_t b = rest / tiles_x;
+  const int x0 = txt * kPsTX, y0 = strip * sy;
+  const int t = threadIdx.x;
+  const int64_t plane = (int64_t)nzpsi * npsi;
+  const bool loader = t < kPsW;
+  int gx = (x0 - H + t) % npsi;
+  gx = gx < 0 ? gx + npsi : gx;
+  const T* src = in + b * plane + gx;
+  int gy = (y0 - H) % nzpsi;
+  gy = gy < 0 ? gy + nzpsi : gy;
+  T win[WIN];
+#pragma unroll
+  for (int k = 0; k < WIN; ++k) {
+    win[k] = loader ? src[(int64_t)gy * npsi] : vzero<T>();
+    gy = gy + 1 == nzpsi ? 0 : gy + 1;
+  }
+  T* dst = out + b * plane;
+  const int ystop = min(sy, nzpsi - y0);
+  const int xr = t / G, xg = t % G;  // x work item: row xr, outputs 8 xg .. 8 xg + 7
+  for (int c = 0; c < ystop; c += R) {
+    if (loader) {
+#pragma unroll
+      for (int o = 0; o < R; ++o) {
+        T acc = vzero<T>();
+#pragma unroll
+        for (int d = 0; d <= 2 * H; ++d) acc = vfma(taps.wy[d], win[o + d], acc);
+        tmp[o * kPsRS + t + t / 8] = acc;
+      }
+    }
+    if (c + R < ystop) {  // slide and fetch the next chunk's rows before the x pass
+#pragma unroll
+      for (int k = 0; k < 2 * H; ++k) win[k] = win[k + R];
+#pragma unroll
+      for (int k = 2 * H; k < WIN; ++k) {
+        win[k] = loader ? src[(int64_t)gy * npsi] : vzero<T>();
+        gy = gy + 1 == nzpsi ? 0 : gy + 1;
+      }
+    }
+    __syncthreads();
+    if (xr < R && c + xr < ystop) {
+      T xw[WIN];
+#pragma unroll
+      for (int k = 0; k < WIN; ++k) {
+        const int col = 8 * xg + k;
+        xw[k] = tmp[xr * kPsRS + col + col / 8];
+      }
+      T* row = dst + (int64_t)(y0 + c + xr) * npsi;
+#pragma unroll
+      for (int o = 0; o < 8; ++o) {
+        const int xx = x0 + 8 * xg + o;
+        if (xx >= npsi) break;
+        T acc = vzero<T>();
+#pragma unroll
+        for (int d = 0; d <= 2 * H; ++d) acc = vfma(taps.wx[d], xw[o + d], acc);
+        row[xx] = acc;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
 template <typename T>
@@ -347,6 +432,19 @@ hipError_t bs_launch_dadj_reduce(BsArgs<T> a, const T* g, float2* partial, float
 template <typename T>
 hipError_t bs_launch_prefilter(const T* in, T* out, int64_t batch, int nzpsi, int npsi,
                                const PrefilterTaps& taps, hipStream_t st) {
+  if (npsi >= kPsTX) {
+    // strips of sy rows (a multiple of kPsR): the tallest that still gives
+    // about 2048 blocks, at least kPsR * 2
+    const int tiles_x = (npsi + kPsTX - 1) / kPsTX;
+    int sy = 128;
+    while (sy > 2 * kPsR && batch * tiles_x * (int64_t)((nzpsi + sy - 1) / sy) < 2048) sy /= 2;
+    const int strips = (nzpsi + sy - 1) / sy;
+    const int64_t grid = batch * (int64_t)tiles_x * strips;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL((bs_prefilter_strip_k<T>), dim3((unsigned)grid), dim3(kPsNT), 0, st, in, out, nzpsi, npsi,
+                       tiles_x, strips, sy, taps);
+    return hipGetLastError();
+  }
   const int tx = (npsi + kPfTX - 1) / kPfTX, ty = (nzpsi + kPfTY - 1) / kPfTY;
   const int64_t grid = batch * (int64_t)tx * ty;
   if (grid == 0) return hipSuccess;

@@ -143,3 +143,17 @@ def test_deterministic(gpu):
     x1 = g.dcurlySadjc(c, dev(r), dev(m), s)
     x2 = g.dcurlySadjc(c, dev(r), dev(m), s)
     assert torch.equal(x1[1], x2[1])
+
+
+@pytest.mark.parametrize("shape", [(2, 300, 520), (3, 6, 256), (1, 1024, 1024), (2, 38, 300)])
+@pytest.mark.parametrize("real", [False, True])
+def test_coeff_wide_planes(gpu, shape, real):
+    """Planes at least 256 wide take the column-strip prefilter: partial
+    panels in x, partial strips in y, planes shorter than the filter reach."""
+    from holotomocupy_amd import Shift
+    print(f"[case] coeff wide planes shape={shape} real={real}", flush=True)
+    B, nzpsi, npsi = shape
+    rng = np.random.default_rng(13)
+    c = data(rng, shape, real)
+    g = Shift(npsi, npsi, nzpsi, nzpsi, "float32" if real else "complex64")
+    assert rel(g.coeff(dev(c)).cpu(), osh.coeff(c, real=real)) < REL_TOL
