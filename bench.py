@@ -413,7 +413,8 @@ def op_bh(a):
 
 def op_serve_mix(a, nreq=200, rate=200.0, seed=1005):
     """Serving mix: Poisson arrivals at `rate` requests/s of operator (D or DT of
-    A fields at one distance), BH gradient and BH bilinear-form requests
+    A fields at one distance, R or RT of an 8-slice slab at A angles), BH
+    gradient and BH bilinear-form requests
     (weights 0.4 / 0.4 / 0.2), detector and object grid n in {256, 512, 1024, 2048}
     (weights 0.4 / 0.3 / 0.2 / 0.1, object n x n), 1-4 distances, 16-128
     angles.  Served: BHServer (admission by HBM budget, batches per bucket, HIP
@@ -424,7 +425,9 @@ def op_serve_mix(a, nreq=200, rate=200.0, seed=1005):
     import torch
     from holotomocupy_amd import phantom
     from holotomocupy_amd.rec import Rec
-    from holotomocupy_amd.serve import BHServer, BilinearRequest, Geometry, GradientRequest, OperatorRequest
+    from holotomocupy_amd.serve import (BHServer, BilinearRequest, Geometry, GradientRequest, OperatorRequest,
+                                        TomoRequest)
+    from holotomocupy_amd.tomo import Tomo
     rng = np.random.default_rng(seed)
     gen = torch.Generator(device="cuda").manual_seed(seed)
     sizes, kinds = [256, 512, 1024, 2048], ["op", "grad", "hess"]
@@ -458,8 +461,14 @@ def op_serve_mix(a, nreq=200, rate=200.0, seed=1005):
     def make(kind, n, nd, A):
         G = geo(n, nd)
         if kind == "op":
+            op = str(rng.choice(["D", "DT", "R", "RT"] if n <= 1024 else ["D", "DT"]))  # Radon plans: n <= 1024
+            if op in ("R", "RT"):  # slab of 8 slices, A uniform angles on [0, pi), no mask
+                theta = tuple(float(t) for t in np.arange(A) * np.pi / A)
+                shape = (8, n, n) if op == "R" else (A, 8, n)
+                x = pool("field", n, int(np.prod(shape)), torch.complex64, 1.0).view(shape)
+                return TomoRequest(n, theta, 0.0, op, x)
             x = pool("field", n, A * n * n, torch.complex64, 1.0).view(A, n, n)
-            return OperatorRequest(G, "D" if rng.random() < 0.5 else "DT", int(rng.integers(0, nd)), x)
+            return OperatorRequest(G, op, int(rng.integers(0, nd)), x)
         proj = pool("proj", n, A * n * n, torch.complex64, 0.3).view(A, n, n)
         pos = 2 * torch.rand((A, nd, 2), device="cuda", generator=gen) - 1
         data = pool("data", n, A * nd * n * n, torch.float32, 0.25, 1.0).view(A, nd, n, n)
@@ -476,6 +485,7 @@ def op_serve_mix(a, nreq=200, rate=200.0, seed=1005):
     say = lambda m: print(f"[serve-mix] {m}", file=sys.stderr, flush=True)  # noqa: E731
     say(f"{nreq} requests made")
     clone = lambda r: (OperatorRequest(r.geometry, r.op, r.j, r.x) if isinstance(r, OperatorRequest) else  # noqa: E731
+                       TomoRequest(r.n, r.theta, r.mask_r, r.op, r.x) if isinstance(r, TomoRequest) else
                        GradientRequest(r.geometry, r.prb, r.proj, r.pos, r.data, r.data_size, r.rho)
                        if isinstance(r, GradientRequest) else
                        BilinearRequest(r.geometry, r.x, r.y, r.z, r.data, r.data_size))
@@ -539,7 +549,14 @@ def op_serve_mix(a, nreq=200, rate=200.0, seed=1005):
                                           obj_dtype="complex64", rho=[1.0, 0.05, 0.02], nchunk=16, **g))
         return recs[G]
 
+    tomos = {}
+
     def plain(r):
+        if isinstance(r, TomoRequest):
+            if (r.n, r.theta) not in tomos:
+                tomos[(r.n, r.theta)] = Tomo(r.n, 1 << 30, np.asarray(r.theta), r.mask_r)
+            t = tomos[(r.n, r.theta)]
+            return t.R(r.x) if r.op == "R" else t.RT(r.x)
         rec = rec_for(r.geometry)
         if isinstance(r, OperatorRequest):
             return (rec.cl_prop.D if r.op == "D" else rec.cl_prop.DT)(r.x, r.j)
@@ -579,8 +596,8 @@ def op_serve_mix(a, nreq=200, rate=200.0, seed=1005):
             "unit": "requests/s", "n_gpus": 1, "steps": nreq, "warmup": 2 * nreq,
             "ms_per_step": round(cap_s * 1e3 / nreq, 3), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "c64", "data": "synthetic (seeded)",
-            "config": {"workload": f"{nreq} requests; n in 256..2048, 1-4 distances, 16-128 angles; kinds "
-                                   f"operator/gradient/bilinear = {counts}; value = all submitted at once; "
+            "config": {"workload": f"{nreq} requests; n in 256..2048, 1-4 distances, 16-128 angles; kinds operator (D/DT/R/RT) "
+                                   f"/ gradient / bilinear = {counts}; value = all submitted at once; "
                                    f"latency at Poisson arrivals of {rate:g}/s"},
             "plain_value": round(nreq / plain_s, 3) if plain_s else None,
             "speedup_vs_plain": round(plain_s / cap_s, 3) if plain_s else None,

@@ -1,9 +1,10 @@
 """Continuous-batching request loop for BH operator solves on one GPU.
 
-Three kinds of requests: BH gradients (the main kind, below), propagations
+Four kinds of requests: BH gradients (the main kind, below), propagations
 D / DT of batches of fields (packed per geometry, operator and distance into
-one launch) and BH bilinear forms (run by the fused bilinear-form kernel, one
-launch sequence per request).  Gradient requests (a chunk of angles of some
+one launch), Radon transforms R / RT of slabs (stacked along z per grid, angle
+set, mask and operator into one launch) and BH bilinear forms (run by the
+fused bilinear-form kernel, one launch sequence per request).  Gradient requests (a chunk of angles of some
 reconstruction: probe, projected object, positions, measured amplitudes,
 geometry) are
   * bucketed by geometry (detector and object grid, distances, physics), since
@@ -103,6 +104,34 @@ class OperatorRequest:
     @property
     def key(self):
         return (self.geometry, "prop", self.op, int(self.j))
+
+
+@dataclass
+class TomoRequest:
+    """R or RT of a slab (Tomo.R / RT, tomo.py:56-116) at angles theta (a
+    tuple of floats) with support mask radius mask_r: x is obj [nz, n, n] for R,
+    data [ntheta, nz, n] for RT (complex64 or float32 device tensors).
+    Requests of one grid, angle set, mask and operator share one launch, their
+    slabs stacked along z; the result is the request's own output."""
+    n: int
+    theta: tuple
+    mask_r: float
+    op: str
+    x: object
+    nangles: int = 0
+    future: Future = field(default_factory=Future)
+    t_submit: float = 0.0
+    t_done: float = 0.0
+    nbytes: int = 0
+
+    def __post_init__(self):
+        if self.op not in ("R", "RT"):
+            raise ValueError("op must be 'R' or 'RT'")
+        self.nangles = int(self.x.shape[0] if self.op == "R" else self.x.shape[1])  # slices
+
+    @property
+    def key(self):
+        return ("tomo", int(self.n), tuple(self.theta), float(self.mask_r), self.op, self.x.dtype)
 
 
 @dataclass
@@ -251,6 +280,7 @@ class GpuExecutor:
         self.buckets = {}
         self.use_graphs = use_graphs
         self.seen = collections.Counter()  # gradient batch shapes run so far
+        self.tomos = {}  # (n, theta, mask_r) -> Tomo plan
         self.arena_growths = 0
 
     # ---- geometry buckets
@@ -273,6 +303,9 @@ class GpuExecutor:
         """Device bytes a request holds while in flight beyond the arenas: the
         packed batch and output of a propagation, the result copies of a
         gradient, the scalar of a bilinear form."""
+        if isinstance(req, TomoRequest):  # stacked input, output, oversampled grid
+            nth = len(req.theta)
+            return req.nangles * (req.n * req.n * 8 + nth * req.n * 8 + 32 * req.n * req.n)
         g = req.geometry
         if isinstance(req, OperatorRequest):
             return 2 * req.nangles * g.nz * g.n * 8
@@ -366,6 +399,35 @@ class GpuExecutor:
             ev.record(s)
         return ev, outs
 
+    def _tomo(self, key):
+        _, n, theta, mask_r = key[:4]
+        if (n, theta, mask_r) not in self.tomos:
+            from .tomo import Tomo
+            self.tomos[(n, theta, mask_r)] = Tomo(n, 1 << 30, np.asarray(theta), mask_r)
+        return self.tomos[(n, theta, mask_r)]
+
+    def _run_tomo(self, key, batch, s):
+        torch = self.torch
+        op = key[4]
+        tomo = self._tomo(key)
+        with torch.cuda.stream(s):
+            for r in batch:
+                r.x.record_stream(s)
+            zdim = 0 if op == "R" else 1
+
+            def apply():
+                x = batch[0].x if len(batch) == 1 else torch.cat([r.x for r in batch], dim=zdim)
+                return tomo.R(x) if op == "R" else tomo.RT(x)
+
+            y = self._alloc(apply)
+            outs, o = [], 0
+            for r in batch:  # R output [ntheta, nz, n]: slabs along dim 1; RT output [nz, n, n]
+                outs.append(y[:, o:o + r.nangles] if op == "R" else y[o:o + r.nangles])
+                o += r.nangles
+            ev = torch.cuda.Event()
+            ev.record(s)
+        return ev, outs
+
     def _run_hess(self, key, batch, s, si):
         torch = self.torch
         geo = key[0]
@@ -395,6 +457,8 @@ class GpuExecutor:
         self.next_stream = (si + 1) % len(self.streams)
         s = self.streams[si]
         if not isinstance(geo, Geometry):  # operator and bilinear-form buckets
+            if geo[0] == "tomo":
+                return self._run_tomo(geo, batch, s)
             return self._run_prop(geo, batch, s) if geo[1] == "prop" else self._run_hess(geo, batch, s, si)
         b = self._bucket(geo)
         A = sum(r.nangles for r in batch)
@@ -467,8 +531,9 @@ class GpuExecutor:
 class BHServer:
     """Thread-backed request loop: submit() returns a Future of the request's
     result: (gprb [ndist, nz, n], gproj [A, nzobj, nobj], gpos [A, ndist, 2])
-    for a GradientRequest, the [B, nz, n] field for an OperatorRequest, a
-    float64 [1] tensor for a BilinearRequest."""
+    for a GradientRequest, the [B, nz, n] field for an OperatorRequest, the
+    request's slab of R / RT output for a TomoRequest, a float64 [1] tensor for
+    a BilinearRequest."""
 
     def __init__(self, hbm_budget=None, max_batch_angles=128, use_graphs=True, executor=None):
         """hbm_budget: device bytes that requests in flight may hold beyond the

@@ -7,7 +7,10 @@ from holotomocupy_amd.serve import BHServer, Geometry, GradientRequest, Schedule
 
 
 class _T:
-    """Stand-in for a device tensor: only .shape is used by the scheduler."""
+    """Stand-in for a device tensor: only .shape and .dtype are used by the
+    scheduler."""
+    dtype = "complex64"
+
     def __init__(self, *shape):
         self.shape = shape
 
@@ -241,3 +244,41 @@ def test_arena_growth_and_graph_capture_keep_results(gpu):
     for a, b in zip(ref, got):
         for x, y in zip(a, b):
             assert torch.equal(x, y)
+
+
+def test_tomo_requests_bucket_by_grid_angles_and_operator():
+    from holotomocupy_amd.serve import TomoRequest
+    ex = FakeExec(per_angle=1)
+    s = Scheduler(ex, hbm_budget=10 ** 9, max_batch_angles=64, max_inflight=8)
+    th = (0.0, 0.5, 1.0)
+    s.submit(TomoRequest(8, th, 0.0, "R", _T(4, 8, 8)))
+    s.submit(TomoRequest(8, th, 0.0, "RT", _T(3, 2, 8)))
+    s.submit(TomoRequest(8, th, 0.0, "R", _T(2, 8, 8)))
+    s.submit(TomoRequest(8, (0.0, 0.5), 0.0, "R", _T(1, 8, 8)))
+    s.step()
+    assert [b for _, b in ex.runs] == [[4, 2], [2], [1]]  # slices per batch
+
+
+@pytest.mark.gpu
+def test_tomo_requests_equal_direct_calls(gpu):
+    """R / RT requests stacked along z in one launch equal Tomo.R / RT of
+    each slab alone bit for bit."""
+    import torch
+    from holotomocupy_amd.serve import TomoRequest
+    from holotomocupy_amd.tomo import Tomo
+    print("[case] serve tomo requests", flush=True)
+    n, nth = 64, 24
+    theta = tuple(float(t) for t in np.arange(nth) * np.pi / nth)
+    g = torch.Generator(device="cuda").manual_seed(79)
+    objs = [torch.randn((k, n, n), dtype=torch.complex64, device="cuda", generator=g) for k in (3, 5)]
+    sinos = [torch.randn((nth, k, n), dtype=torch.complex64, device="cuda", generator=g) for k in (2, 4)]
+    srv = BHServer(hbm_budget=8 << 30)
+    reqs = [TomoRequest(n, theta, 0.9, "R", o) for o in objs] + [TomoRequest(n, theta, 0.9, "RT", d) for d in sinos]
+    futs = [srv.submit(r) for r in reqs]
+    res = [f.result(timeout=60) for f in futs]
+    srv.close()
+    t = Tomo(n, 16, np.asarray(theta), 0.9)
+    for o, got in zip(objs, res[:2]):
+        assert torch.equal(got, t.R(o))
+    for d, got in zip(sinos, res[2:]):
+        assert torch.equal(got, t.RT(d))
