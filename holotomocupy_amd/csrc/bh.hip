@@ -378,58 +378,104 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
 }
 
 // ---- K4: gather-form Sadj summed over distances ----------------------------
+// K4: dpsi = rho_obj * sum_j Sadj_j(T_j) as a gather.  A workgroup owns a
+// 64 x kSadjTY tile of coefficients: lane l of wave w computes column l of rows
+// w*kSadjR .. w*kSadjR + kSadjR - 1, so its x weights serve kSadjR outputs and
+// a wave's rows are wave-uniform.  For a distance with magnification >= 1
+// every candidate bracket has fewer than 8 entries; the tile's y weights
+// (kSadjTY rows x 8 candidates) are then evaluated once into LDS and read as
+// broadcasts.  Smaller magnifications take the per-thread candidate walk.
+// Either way each output sums over detector rows in ascending order and the
+// distances in order.
+constexpr int kSadjR = 4, kSadjTY = 4 * kSadjR, kSadjK = 8;
+
 template <typename TO>
 __global__ __launch_bounds__(256) void bh_sadj_k(BhArgs a, TO* __restrict__ dpsi, int tiles_x,
                                                  int tiles_y) {
-  constexpr int TX = 64, TY = 4;
+  constexpr int TX = 64, R = kSadjR, TY = kSadjTY, K = kSadjK;
+  __shared__ float s_wy[TY][K];
+  __shared__ int s_ylo[TY];
   const int64_t bid = xcd_tile(blockIdx.x, gridDim.x);
   const int txt = (int)(bid % tiles_x);
   const int64_t rest = bid / tiles_x;
   const int tyt = (int)(rest % tiles_y);
   const int64_t b = rest / tiles_y;
-  const int ix = txt * TX + (int)(threadIdx.x % TX);
-  const int iy = tyt * TY + (int)(threadIdx.x / TX);
-  if (ix >= a.npsi || iy >= a.nzpsi) return;
-  constexpr int kMaxCand = 16;
-  TO acc = vzero<TO>();
+  const int t = threadIdx.x;
+  const int ix = txt * TX + (t & 63);
+  const int r0 = (t >> 6) * R;  // first tile row of this wave
+  const bool xin = ix < a.npsi;
+  TO acc[R];
+#pragma unroll
+  for (int rr = 0; rr < R; ++rr) acc[rr] = vzero<TO>();
   for (int j = 0; j < a.ndist; ++j) {
     const int64_t bj = b * a.ndist + j;
     const float mb = a.mag[bj], ry = a.pos[2 * bj], rx = a.pos[2 * bj + 1];
-    int xlo, xhi, ylo, yhi;
-    bs_candidates(ix, a.n, a.npsi, mb, rx, xlo, xhi);
-    bs_candidates(iy, a.nz, a.nzpsi, mb, ry, ylo, yhi);
     const float2* g = a.T + bj * (int64_t)a.nz * a.n;
-    if (xhi - xlo < kMaxCand) {
-      // x weights once per distance (they do not depend on the detector row)
-      float wxs[kMaxCand];
-#pragma unroll
-      for (int k = 0; k < kMaxCand; ++k)
-        wxs[k] = xlo + k <= xhi ? bs_reach(bs_coord(xlo + k, a.n, a.npsi, mb, rx), ix) : 0.0f;
-      for (int ty = ylo; ty <= yhi; ++ty) {
-        const float wyv = bs_reach(bs_coord(ty, a.nz, a.nzpsi, mb, ry), iy);
-        if (wyv == 0.0f) continue;
-        const float2* gr = g + (int64_t)ty * a.n + xlo;
-        TO rowv = vzero<TO>();
-#pragma unroll
-        for (int k = 0; k < kMaxCand; ++k)
-          if (wxs[k] != 0.0f) rowv = vfma(wxs[k], from_c<TO>(gr[k]), rowv);
-        acc = vfma(wyv, rowv, acc);
-      }
-    } else {
-      for (int ty = ylo; ty <= yhi; ++ty) {
-        const float wyv = bs_reach(bs_coord(ty, a.nz, a.nzpsi, mb, ry), iy);
-        if (wyv == 0.0f) continue;
-        TO rowv = vzero<TO>();
-        for (int tx = xlo; tx <= xhi; ++tx) {
-          const float wxv = bs_reach(bs_coord(tx, a.n, a.npsi, mb, rx), ix);
-          if (wxv != 0.0f) rowv = vfma(wxv, from_c<TO>(g[(int64_t)ty * a.n + tx]), rowv);
+    if (mb >= 1.0f) {  // uniform over the workgroup (one angle)
+      __syncthreads();  // the previous distance's table has been read
+      if (t < TY * K) {
+        const int row = t / K, k = t % K, iy = tyt * TY + row;
+        float w = 0.0f;
+        int ylo = 0, yhi = -1;
+        if (iy < a.nzpsi) {
+          bs_candidates(iy, a.nz, a.nzpsi, mb, ry, ylo, yhi);
+          if (ylo + k <= yhi) w = bs_reach(bs_coord(ylo + k, a.nz, a.nzpsi, mb, ry), iy);
         }
-        acc = vfma(wyv, rowv, acc);
+        s_wy[row][k] = w;
+        if (k == 0) s_ylo[row] = ylo;
+      }
+      __syncthreads();
+      if (xin) {
+        int xlo, xhi;
+        bs_candidates(ix, a.n, a.npsi, mb, rx, xlo, xhi);
+        float wxs[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          wxs[k] = xlo + k <= xhi ? bs_reach(bs_coord(xlo + k, a.n, a.npsi, mb, rx), ix) : 0.0f;
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+          const int ylo = s_ylo[r0 + rr];
+          for (int k = 0; k < K; ++k) {
+            const float wyv = s_wy[r0 + rr][k];
+            if (wyv == 0.0f) continue;  // wave-uniform: the row is the wave's
+            const float2* gr = g + (int64_t)(ylo + k) * a.n + xlo;
+            TO rowv = vzero<TO>();
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk)
+              if (wxs[kk] != 0.0f) rowv = vfma(wxs[kk], from_c<TO>(gr[kk]), rowv);
+            acc[rr] = vfma(wyv, rowv, acc[rr]);
+          }
+        }
+      }
+    } else if (xin) {
+      int xlo, xhi;
+      bs_candidates(ix, a.n, a.npsi, mb, rx, xlo, xhi);
+#pragma unroll
+      for (int rr = 0; rr < R; ++rr) {
+        const int iy = tyt * TY + r0 + rr;
+        if (iy >= a.nzpsi) continue;
+        int ylo, yhi;
+        bs_candidates(iy, a.nz, a.nzpsi, mb, ry, ylo, yhi);
+        for (int ty = ylo; ty <= yhi; ++ty) {
+          const float wyv = bs_reach(bs_coord(ty, a.nz, a.nzpsi, mb, ry), iy);
+          if (wyv == 0.0f) continue;
+          TO rowv = vzero<TO>();
+          for (int tx = xlo; tx <= xhi; ++tx) {
+            const float wxv = bs_reach(bs_coord(tx, a.n, a.npsi, mb, rx), ix);
+            if (wxv != 0.0f) rowv = vfma(wxv, from_c<TO>(g[(int64_t)ty * a.n + tx]), rowv);
+          }
+          acc[rr] = vfma(wyv, rowv, acc[rr]);
+        }
       }
     }
   }
+  if (!xin) return;
   const float rho = a.scales[b].y;
-  dpsi[(b * a.nzpsi + iy) * (int64_t)a.npsi + ix] = vfma(rho, acc, vzero<TO>());
+#pragma unroll
+  for (int rr = 0; rr < R; ++rr) {
+    const int iy = tyt * TY + r0 + rr;
+    if (iy < a.nzpsi) dpsi[(b * a.nzpsi + iy) * (int64_t)a.npsi + ix] = vfma(rho, acc[rr], vzero<TO>());
+  }
 }
 
 // ---- small reductions -------------------------------------------------------
@@ -656,8 +702,7 @@ static hipError_t bh_run(const htamd_shift_plan* sp, BhArgs a, const void* proj,
   sev.mark(6, st);
   if ((e = rows_dispatch<TO>(true, a, st)) != hipSuccess) return e;
   sev.mark(7, st);
-  constexpr int TX = 64, TY = 4;
-  const int tx = (a.npsi + TX - 1) / TX, ty = (a.nzpsi + TY - 1) / TY;
+  const int tx = (a.npsi + 63) / 64, ty = (a.nzpsi + kSadjTY - 1) / kSadjTY;
   TO* dpsi = (TO*)(ws + L.dpsi);
   sev.mark(8, st);
   hipLaunchKernelGGL((bh_sadj_k<TO>), dim3((unsigned)(a.nang * tx * ty)), dim3(256), 0, st, a, dpsi, tx, ty);

@@ -111,3 +111,22 @@ def test_fused_gradient_nonsquare_detector(gpu):
     assert rel(g["prb"], gprb) < 1e-5
     assert rel(g["proj"], gproj) < 1e-5
     assert rel(g["pos"], gpos) < 1e-4
+
+
+@pytest.mark.parametrize("real", [False, True])
+def test_fused_gradient_magnification_below_one(gpu, real):
+    """A negative shrink puts some magnifications below 1 (wider candidate
+    brackets in the gather-form Sadj); the rest stay at or above 1."""
+    print(f"[case] fused gradient with magnifications below one real={real}", flush=True)
+    B, nd = 3, 3
+    shrink = np.array([[-0.3, 0.0, 0.1], [-0.15, -0.4, 0.0], [0.2, -0.05, -0.35]], dtype=np.float64)
+    rec, cas, x, v, w, d = make_problem(real, n=32, nobj=40, ndist=nd, B=B, seed=31, shrink=shrink)
+    assert float(rec.eff_demagnifications.min()) < 1.0 <= float(rec.eff_demagnifications.max())
+    xd, dd = dev(x), torch.from_numpy(d).cuda()
+    grads = {"prb": torch.zeros_like(xd["prb"]), "proj": torch.empty_like(xd["proj"]),
+             "pos": torch.empty_like(xd["pos"])}
+    rec.gradients_fused(xd, grads, dd)
+    gprb, gproj, gpos = cas.gradients(*lst(x), d)
+    assert rel(grads["prb"], gprb) < 1e-5
+    assert rel(grads["proj"], gproj) < 1e-5
+    assert rel(grads["pos"], gpos) < 1e-4

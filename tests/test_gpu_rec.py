@@ -24,7 +24,7 @@ def crandn(rng, shape):
     return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
 
 
-def make_problem(real=False, n=32, nobj=40, ndist=3, B=4, seed=5):
+def make_problem(real=False, n=32, nobj=40, ndist=3, B=4, seed=5, shrink=None):
     from holotomocupy_amd.rec import Rec, physics
     rng = np.random.default_rng(seed)
     args = SimpleNamespace(energy=17.1, detector_pixelsize=1.4760147601476e-6 * 16,
@@ -32,11 +32,15 @@ def make_problem(real=False, n=32, nobj=40, ndist=3, B=4, seed=5):
                            z1=np.array([5.110, 5.464, 6.879, 9.817])[:ndist] * 1e-3,
                            ndist=ndist, ntheta=B, nz=n, n=n, nzobj=nobj, nobj=nobj,
                            obj_dtype="float32" if real else "complex64", rho=[1, 0.5, 0.2])
+    if shrink is not None:  # per-angle, per-distance magnification change (rec_mpi.py:146)
+        args.shrink = shrink
     rec = Rec(args)
     ph = physics(args.energy, args.z1, args.focustodetectordistance, args.detector_pixelsize)
     prop = op.Propagation(n, n, B, ndist, ph.wavelength, ph.voxelsize, ph.distance)
     sh = osh.Shift(n, nobj, n, nobj, args.obj_dtype)
     mag = np.tile(1 / ph.norm_magnifications, (B, 1))
+    if shrink is not None:
+        mag = (1 + np.asarray(shrink)) / ph.norm_magnifications[None, :]
     cas = oc.Cascade(prop, sh, ndist, n, n, nobj, nobj, B * ndist * n * n, mag, rho=args.rho, obj_real=real)
     yy, xx = np.mgrid[0:n, 0:n] - n / 2
     prb = np.stack([np.exp(-(xx ** 2 + yy ** 2) / (2 * (0.4 * n) ** 2)) * np.exp(0.3j * np.cos(0.2 * xx + j))
