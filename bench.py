@@ -41,6 +41,9 @@ def parse():
                         "256^2 at 4 distances; tomo = R and RT at 256^3, 384 angles; shift = coeff, S and "
                         "Sadj at 384 x 256^2; bh = Rec.BH iterations at 512^3, 720 angles, 1024^2; serve-mix = "
                         "Poisson mix of operator, gradient and bilinear-form requests (n 256..2048)")
+    p.add_argument("--bh-config", choices=["c3", "c4"], default="c3",
+                   help="--op bh sizes: c3 = object 512^3, 720 angles, 1024^2 detector; c4 = object 1024^3, "
+                        "1800 angles, 2048^2 detector (4 distances both)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="first run a 256 MiB device-to-device copy 3 times (a known-size kernel for "
                         "scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -350,18 +353,21 @@ def op_shift(a):
 
 
 def op_bh(a):
-    """Rec.BH iterations on one GPU at 512^3 object, 720 angles, 1024^2 detector,
-    4 distances (16-angle launches, 16-slice tomography chunks, Laplacian
-    regulariser on).  One warm-up iteration (it has no beta step), then
-    `steps` timed iterations; the phases are timed with events on the launch
-    stream.  Bytes: the gradient's model bytes per angle times the angles, for
-    the roofline of the whole iteration."""
+    """Rec.BH iterations on one GPU (16-angle launches, 16-slice tomography
+    chunks, Laplacian regulariser on): `--bh-config c3` is object 512^3, 720
+    angles, detector 1024^2, 4 distances; `c4` is one rank's share of the
+    largest case, object 1024^3, 1800 angles, detector 2048^2, 4 distances
+    (121 GB of data, every array resident in HBM).  One warm-up iteration (it
+    has no beta step), then `steps` timed iterations; the phases are timed with
+    events on the launch stream.  Bytes: the gradient's model bytes per angle
+    times the angles, for the roofline of the whole iteration."""
     import numpy as np
     import torch
     from types import SimpleNamespace
     from holotomocupy_amd import phantom
     from holotomocupy_amd.rec import Rec
-    n, nobj, nth, nd = 1024, 512, 720, 4
+    n, nobj, nth, nd = (2048, 1024, 1800, 4) if a.bh_config == "c4" else (1024, 512, 720, 4)
+    say = lambda m: print(f"[bh {a.bh_config}] {m}", file=sys.stderr, flush=True)  # noqa: E731
     g = geometry(n, nd)
     theta = np.arange(nth) * np.pi / nth
     args = SimpleNamespace(ndist=nd, ntheta=nth, nz=n, n=n, nzobj=nobj, nobj=nobj, obj_dtype="complex64",
@@ -381,13 +387,16 @@ def op_bh(a):
     del zz, yy, xx
     obj = torch.complex(-delta, delta / 100)
     del delta
+    say("phantom made")
     proj = torch.empty((nth, nobj, nobj), dtype=torch.complex64, device="cuda")
     rec.fwd_tomo(obj / float(rec.norm_const), out=proj)
     obj *= 1.0 / float(proj.abs().max())  # phase of order one radian
     del proj
+    say("phantom scaled")
     prb = torch.from_numpy(phantom.probe(rng, nd, n, n)).cuda()
     pos = torch.from_numpy(phantom.positions(rng, nth, nd)).cuda()
     rec.gen_sqrt_data({"obj": obj, "prb": prb, "pos": pos})
+    say(f"data made; {torch.cuda.memory_allocated() / 2**30:.1f} GiB allocated")
     rec.vars["obj"].copy_(obj * 0.8)
     rec.vars["prb"].copy_(prb * 0.9)
     rec.vars["pos"].copy_(pos + 0.3 * torch.randn(pos.shape, generator=gen, device="cuda"))
@@ -398,14 +407,18 @@ def op_bh(a):
     rec.BH()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    say(f"{1 + a.steps} iterations in {wall:.2f} s; peak {torch.cuda.max_memory_allocated() / 2**30:.1f} GiB")
     timed = rec.phase_ms[1:]
     parts = {k: sum(p[k] for p in timed) / len(timed) for k in timed[0]}
     ms = sum(parts.values())
     F, Fo = 8 * n * n, 8 * nobj * nobj
     nbytes = nth * (nd * (6.5 * F + 2 * Fo) + 9 * Fo)
-    line = _op_line("BH iterations/s (object 512^3, 720 angles, 1024^2 detector, 4 distances)", 1e3 / ms,
-                    "iterations/s", ms, nbytes, "Rec.BH: gradient, R of the gradient, three bilinear forms, "
-                    "updates; object 512^3, 720 angles, detector 1024^2, 4 distances, Laplacian on", a, parts)
+    shape = f"object {nobj}^3, {nth} angles, {n}^2 detector, {nd} distances"
+    line = _op_line(f"BH iterations/s ({shape})", 1e3 / ms, "iterations/s", ms, nbytes,
+                    f"Rec.BH: gradient, R of the gradient, three bilinear forms, updates; {shape}, Laplacian on",
+                    a, parts)
+    line["warmup"] = 1  # the first iteration (no beta step) is the warm-up
+    line["peak_hbm_gib"] = round(torch.cuda.max_memory_allocated() / 2**30, 1)
     line["wall_s_all_iterations"] = round(wall, 3)
     line["alphas"] = [float(x) for x in rec.alphas]
     return line
