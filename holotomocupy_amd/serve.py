@@ -182,6 +182,9 @@ class Scheduler:
         self.rejected = 0
 
     def submit(self, req):
+        prepare = getattr(self.ex, "prepare", None)
+        if prepare is not None:
+            prepare(req)
         req.nbytes = int(self.ex.footprint(req))
         if req.nbytes > self.budget:
             self.rejected += 1
@@ -518,13 +521,25 @@ class GpuExecutor:
             ev.record(s)
         return ev, outs
 
+    def prepare(self, req):
+        """Called in the submitting thread: its current stream is where the
+        request's results will be used."""
+        req.consumer = self.torch.cuda.current_stream(self.device)
+
     def done(self, handle):
         return handle[0].query()
 
     def finish(self, handle, batch):
+        """Results were allocated on an executor stream; recording them on the
+        consumer's stream keeps the allocator from handing their blocks to a
+        later batch while the consumer's kernels may still read them."""
         ev, outs = handle
         ev.synchronize()
         for r, res in zip(batch, outs):
+            consumer = getattr(r, "consumer", None)
+            if consumer is not None:
+                for t in (res if isinstance(res, tuple) else (res,)):
+                    t.record_stream(consumer)
             r.future.set_result(res)
 
 
