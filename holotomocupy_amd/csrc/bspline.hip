@@ -113,36 +113,98 @@ __global__ __launch_bounds__(kTX* kTY) void bs_forward_k(BsArgs<T> a) {
 //   sum_{ty: iy in taps(y(ty))} wy(ty) sum_{tx: ix in taps(x(tx))} wx(tx) g[b, ty, tx].
 // Candidate pixels are bracketed analytically with a one-pixel margin and each
 // is re-evaluated with the forward's exact float expression, so the weights
-// are bitwise those of S.
+// are bitwise those of S.  Tiles of 64 x kAdjTY coefficients: lane l of wave w
+// computes column l of rows w*kAdjR .. w*kAdjR + kAdjR - 1, so its x weights
+// serve kAdjR outputs.  With magnification >= 1 every candidate bracket holds
+// fewer than 8 entries (an entry past 8 is the outer margin, whose weight is
+// zero), and the tile's y weights are evaluated once into LDS; smaller
+// magnifications walk the brackets per thread.  Each output sums detector rows
+// in ascending order either way.
+constexpr int kAdjR = 4, kAdjTY = 4 * kAdjR, kAdjK = 8;
+
 template <typename T>
-__global__ __launch_bounds__(kTX* kTY) void bs_adjoint_k(BsArgs<T> a) {
+__global__ __launch_bounds__(256) void bs_adjoint_k(BsArgs<T> a) {
   // here a.c is the input g [B, nz, n] and a.out the coefficients [B, nzpsi, npsi]
+  constexpr int R = kAdjR, TY = kAdjTY, K = kAdjK;
+  __shared__ float s_wy[TY][K];
+  __shared__ int s_ylo[TY];
   const int64_t bid = blockIdx.x;
   const int txt = (int)(bid % a.tiles_x);
   const int64_t rest = bid / a.tiles_x;
   const int tyt = (int)(rest % a.tiles_y);
   const int64_t b = rest / a.tiles_y;
-  const int ix = txt * kTX + (int)(threadIdx.x % kTX);
-  const int iy = tyt * kTY + (int)(threadIdx.x / kTX);
-  if (ix >= a.npsi || iy >= a.nzpsi) return;
+  const int t = threadIdx.x;
+  const int ix = txt * 64 + (t & 63);
+  const int r0 = (t >> 6) * R;
+  const bool xin = ix < a.npsi;
   const float mb = a.m[b * a.ms];
   const float rx = a.r[b * a.rs + 1], ry = a.r[b * a.rs + 0];
-  int xlo, xhi, ylo, yhi;
-  bs_candidates(ix, a.n, a.npsi, mb, rx, xlo, xhi);
-  bs_candidates(iy, a.nz, a.nzpsi, mb, ry, ylo, yhi);
   const T* g = a.c + b * (int64_t)a.nz * a.n;
-  T acc = vzero<T>();
-  for (int ty = ylo; ty <= yhi; ++ty) {
-    const float wyv = bs_reach(bs_coord(ty, a.nz, a.nzpsi, mb, ry), iy);
-    if (wyv == 0.0f) continue;
-    T row = vzero<T>();
-    for (int tx = xlo; tx <= xhi; ++tx) {
-      const float wxv = bs_reach(bs_coord(tx, a.n, a.npsi, mb, rx), ix);
-      if (wxv != 0.0f) row = vfma(wxv, g[(int64_t)ty * a.n + tx], row);
+  T acc[R];
+#pragma unroll
+  for (int rr = 0; rr < R; ++rr) acc[rr] = vzero<T>();
+  if (mb >= 1.0f) {  // uniform over the workgroup (one angle)
+    if (t < TY * K) {
+      const int row = t / K, k = t % K, iy = tyt * TY + row;
+      float w = 0.0f;
+      int ylo = 0, yhi = -1;
+      if (iy < a.nzpsi) {
+        bs_candidates(iy, a.nz, a.nzpsi, mb, ry, ylo, yhi);
+        if (ylo + k <= yhi) w = bs_reach(bs_coord(ylo + k, a.nz, a.nzpsi, mb, ry), iy);
+      }
+      s_wy[row][k] = w;
+      if (k == 0) s_ylo[row] = ylo;
     }
-    acc = vfma(wyv, row, acc);
+    __syncthreads();
+    if (xin) {
+      int xlo, xhi;
+      bs_candidates(ix, a.n, a.npsi, mb, rx, xlo, xhi);
+      float wxs[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        wxs[k] = xlo + k <= xhi ? bs_reach(bs_coord(xlo + k, a.n, a.npsi, mb, rx), ix) : 0.0f;
+#pragma unroll
+      for (int rr = 0; rr < R; ++rr) {
+        const int ylo = s_ylo[r0 + rr];
+        for (int k = 0; k < K; ++k) {
+          const float wyv = s_wy[r0 + rr][k];
+          if (wyv == 0.0f) continue;  // wave-uniform: the row is the wave's
+          const T* gr = g + (int64_t)(ylo + k) * a.n + xlo;
+          T row = vzero<T>();
+#pragma unroll
+          for (int kk = 0; kk < K; ++kk)
+            if (wxs[kk] != 0.0f) row = vfma(wxs[kk], gr[kk], row);
+          acc[rr] = vfma(wyv, row, acc[rr]);
+        }
+      }
+    }
+  } else if (xin) {
+    int xlo, xhi;
+    bs_candidates(ix, a.n, a.npsi, mb, rx, xlo, xhi);
+#pragma unroll
+    for (int rr = 0; rr < R; ++rr) {
+      const int iy = tyt * TY + r0 + rr;
+      if (iy >= a.nzpsi) continue;
+      int ylo, yhi;
+      bs_candidates(iy, a.nz, a.nzpsi, mb, ry, ylo, yhi);
+      for (int ty = ylo; ty <= yhi; ++ty) {
+        const float wyv = bs_reach(bs_coord(ty, a.nz, a.nzpsi, mb, ry), iy);
+        if (wyv == 0.0f) continue;
+        T row = vzero<T>();
+        for (int tx = xlo; tx <= xhi; ++tx) {
+          const float wxv = bs_reach(bs_coord(tx, a.n, a.npsi, mb, rx), ix);
+          if (wxv != 0.0f) row = vfma(wxv, g[(int64_t)ty * a.n + tx], row);
+        }
+        acc[rr] = vfma(wyv, row, acc[rr]);
+      }
+    }
   }
-  a.out[(b * a.nzpsi + iy) * (int64_t)a.npsi + ix] = acc;
+  if (!xin) return;
+#pragma unroll
+  for (int rr = 0; rr < R; ++rr) {
+    const int iy = tyt * TY + r0 + rr;
+    if (iy < a.nzpsi) a.out[(b * a.nzpsi + iy) * (int64_t)a.npsi + ix] = acc[rr];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -399,10 +461,11 @@ hipError_t bs_launch_forward(BsArgs<T> a, int mode, hipStream_t st) {
 
 template <typename T>
 hipError_t bs_launch_adjoint(BsArgs<T> a, hipStream_t st) {
-  tiles_of(a, true);
+  a.tiles_x = (a.npsi + 63) / 64;
+  a.tiles_y = (a.nzpsi + kAdjTY - 1) / kAdjTY;
   const int64_t grid = a.batch * (int64_t)a.tiles_x * a.tiles_y;
   if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL((bs_adjoint_k<T>), dim3((unsigned)grid), dim3(kTX * kTY), 0, st, a);
+  hipLaunchKernelGGL((bs_adjoint_k<T>), dim3((unsigned)grid), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
