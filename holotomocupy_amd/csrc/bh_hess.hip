@@ -378,6 +378,29 @@ __global__ __launch_bounds__(C*(N / E)) void bhh_cols_k(BhhArgs a, int tiles_per
   }
 }
 
+// First stage of the partial sums: block g of nb sums tiles
+// [g * chunk, min(count, (g + 1) * chunk)) in a fixed order into
+// part2[g * nout + k]; bhh_sum_k then sums the nb block sums.
+constexpr int kSumBlocks = 256;
+
+__global__ __launch_bounds__(256) void bhh_sum1_k(const double* __restrict__ part, int64_t count, int nout,
+                                                  int64_t chunk, double* __restrict__ part2) {
+  __shared__ double r[256];
+  const int64_t lo = blockIdx.x * chunk, hi = lo + chunk < count ? lo + chunk : count;
+  for (int k = 0; k < nout; ++k) {
+    double s = 0.0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += 256) s += part[i * nout + k];
+    r[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) r[threadIdx.x] += r[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part2[blockIdx.x * nout + k] = r[0];
+    __syncthreads();
+  }
+}
+
 // out[k] (+)= scale * sum over tiles t of part[t * nout + k], fixed order
 __global__ __launch_bounds__(256) void bhh_sum_k(const double* __restrict__ part, int64_t count, int nout,
                                                  double scale, int accumulate, double* __restrict__ out) {
@@ -426,7 +449,7 @@ static int hess_cols_c(int64_t nz) {
 }
 
 struct BhhLayout {
-  int64_t c = 0, cy = 0, cz = 0, F[5] = {0, 0, 0, 0, 0}, part = 0, total = 0;
+  int64_t c = 0, cy = 0, cz = 0, F[5] = {0, 0, 0, 0, 0}, part = 0, part2 = 0, total = 0;
 };
 
 static BhhLayout bhh_layout(const htamd_prop_plan* pp, const htamd_shift_plan* sp, int64_t nang, int mode) {
@@ -454,6 +477,8 @@ static BhhLayout bhh_layout(const htamd_prop_plan* pp, const htamd_shift_plan* s
   const int64_t ntiles = nang * pp->ndist * (pp->n / C);
   L.part = off;
   off += align256(ntiles * 8 * (pair ? 2 : 1));
+  L.part2 = off;
+  off += align256(kSumBlocks * 8 * (pair ? 2 : 1));
   L.total = off;
   return L;
 }
@@ -547,8 +572,13 @@ static hipError_t bhh_run(const htamd_shift_plan* sp, int mode, BhhArgs a, const
   if ((e = hess_rows_dispatch<TO>(mode, a, st)) != hipSuccess) return e;
   if ((e = hess_cols_dispatch(mode, a, st)) != hipSuccess) return e;
   const int64_t ntiles = a.nang * a.ndist * (a.n / hess_cols_c(a.nz));
-  hipLaunchKernelGGL(bhh_sum_k, dim3(1), dim3(256), 0, st, a.part, ntiles, mode == kPair ? 2 : 1, scale,
-                     accumulate, result);
+  const int nout = mode == kPair ? 2 : 1;
+  const int64_t chunk = (ntiles + kSumBlocks - 1) / kSumBlocks;
+  const int64_t nb = (ntiles + chunk - 1) / chunk;  // <= kSumBlocks
+  double* part2 = (double*)(ws + L.part2);
+  hipLaunchKernelGGL(bhh_sum1_k, dim3((unsigned)nb), dim3(256), 0, st, a.part, ntiles, nout, chunk, part2);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(bhh_sum_k, dim3(1), dim3(256), 0, st, part2, nb, nout, scale, accumulate, result);
   return hipGetLastError();
 }
 
