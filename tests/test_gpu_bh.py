@@ -130,3 +130,48 @@ def test_fused_gradient_magnification_below_one(gpu, real):
     assert rel(grads["prb"], gprb) < 1e-5
     assert rel(grads["proj"], gproj) < 1e-5
     assert rel(grads["pos"], gpos) < 1e-4
+
+
+def test_fused_gradient_bench_size_properties(gpu):
+    """Detector 2048^2, object plane 1024^2, 4 distances, 4 angles (the bench's
+    shapes): the fused gradient agrees with the per-operator cascade (a second,
+    independent implementation) to the operator bar, is deterministic, and its
+    per-angle outputs do not depend on how the angles are split across
+    launches (the probe gradient, a sum over angles, to 1e-6)."""
+    print("[case] fused gradient 2048^2 properties", flush=True)
+    from types import SimpleNamespace
+    from holotomocupy_amd import phantom
+    from holotomocupy_amd.rec import Rec
+    n, nobj, nd, B = 2048, 1024, 4, 4
+    rng = np.random.default_rng(19)
+    args = SimpleNamespace(energy=17.1, detector_pixelsize=1.4760147601476e-6 * 2, focustodetectordistance=1.217,
+                           z1=np.array([5.110, 5.464, 6.879, 9.817]) * 1e-3, ndist=nd, ntheta=B, nz=n, n=n,
+                           nzobj=nobj, nobj=nobj, obj_dtype="complex64", rho=[1.0, 0.05, 0.02])
+    rec = Rec(args)
+    x = {"prb": torch.from_numpy(phantom.probe(rng, nd, n, n)).cuda(),
+         "proj": torch.from_numpy(phantom.projections(rng, B, nobj, nobj)).cuda(),
+         "pos": torch.from_numpy(phantom.positions(rng, B, nd)).cuda()}
+    with torch.no_grad():
+        data = (rec.forward(x).abs() * (1 + 0.05 * torch.randn((B, nd, n, n), device="cuda"))).float().contiguous()
+
+    def zeros():
+        return {"prb": torch.zeros_like(x["prb"]), "proj": torch.empty_like(x["proj"]),
+                "pos": torch.empty_like(x["pos"])}
+
+    g1, g2, gc = zeros(), zeros(), zeros()
+    rec.gradients_fused(x, g1, data)
+    rec.gradients_fused(x, g2, data)
+    for k in g1:
+        assert torch.equal(g1[k], g2[k]), k
+    rec.gradients_cascade(x, gc, data)
+    print({k: rel(g1[k], gc[k].cpu().numpy()) for k in g1}, flush=True)
+    assert rel(g1["prb"], gc["prb"].cpu().numpy()) < 1e-5
+    assert rel(g1["proj"], gc["proj"].cpu().numpy()) < 1e-5
+    assert rel(g1["pos"], gc["pos"].cpu().numpy()) < 1e-4
+    gs = zeros()
+    for st in (0, 2):
+        part = {"prb": x["prb"], "proj": x["proj"][st:st + 2], "pos": x["pos"][st:st + 2]}
+        out = {"prb": gs["prb"], "proj": gs["proj"][st:st + 2], "pos": gs["pos"][st:st + 2]}
+        rec.gradients_fused(part, out, data[st:st + 2], st=st)
+    assert torch.equal(gs["proj"], g1["proj"]) and torch.equal(gs["pos"], g1["pos"])
+    assert rel(gs["prb"], g1["prb"].cpu().numpy()) < 1e-6
