@@ -227,15 +227,18 @@ def roofline_fused(a, rec, vars_, grads, data, steps=3):
             "pass_ms": {s: round(t, 3) for s, t in zip(STAGES, tot)}}
 
 
-def cpu_baseline(a, host):
+def cpu_baseline(a, host, vars_, data):
     """Oracle (NumPy/SciPy port of the reference cascade) on one angle of the
-    same workload: one BH gradient evaluation, timed once."""
+    same workload: one BH gradient evaluation at the timed variables and data
+    (angle 0), timed once.  Returns the baseline record and (inputs, float32
+    data, FP64 gradients) for the parity check of the GPU path on the same
+    sample."""
     import numpy as np
     from oracle import cascade as oc
     from oracle import propagation as op
     from oracle import shift as osh
     from holotomocupy_amd.rec import physics
-    prb, proj, pos, args, g = host
+    _, _, _, args, g = host
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     cores = max(1, min(cores, os.cpu_count() or 1))
     op.WORKERS = cores
@@ -244,15 +247,44 @@ def cpu_baseline(a, host):
     prop = op.Propagation(a.n, a.n, 1, a.ndist, ph.wavelength, ph.voxelsize, ph.distance)
     sh = osh.Shift(a.n, a.nobj, a.n, a.nobj)
     mag = (1 / ph.norm_magnifications)[None, :]
-    cas = oc.Cascade(prop, sh, a.ndist, a.n, a.n, a.nobj, a.nobj, a.angles * a.ndist * a.n * a.n, mag)
-    x = [prb.astype(np.complex128), proj[:1].astype(np.complex128), pos[:1].astype(np.float64)]
-    d = np.abs(cas.forward(*x)).astype(np.float64)
+    cas = oc.Cascade(prop, sh, a.ndist, a.n, a.n, a.nobj, a.nobj, a.angles * a.ndist * a.n * a.n, mag,
+                     rho=tuple(args.rho))
+    prb = vars_["prb"].cpu().numpy()
+    proj = vars_["proj"][:1].cpu().numpy()
+    pos = vars_["pos"][:1].cpu().numpy()
+    d = data[:1].cpu().numpy()
+    x = [prb.astype(np.complex128), proj.astype(np.complex128), pos.astype(np.float64)]
     t0 = time.perf_counter()
-    cas.gradients(*x, d)
+    ref = cas.gradients(*x, d.astype(np.float64))
     dt = time.perf_counter() - t0
-    return {"value": round(1.0 / dt, 5), "unit": "evals/s", "cores": cores, "kind": "port",
+    base = {"value": round(1.0 / dt, 5), "unit": "evals/s", "cores": cores, "kind": "port",
             "sample": f"1 angle x {a.ndist} distances at {a.n}^2 / object {a.nobj}^2, FP64 NumPy/SciPy "
                       f"oracle cascade (scipy.fft workers={cores}), one run of {dt:.1f} s"}
+    return base, (prb, proj, pos, d, ref)
+
+
+def parity_vs_fp64(rec, sample):
+    """Relative L2 error against the FP64 oracle on the cpu_baseline sample
+    (same inputs, same float32 data) of the fused GPU gradient and, for
+    comparison, of the per-operator FP32 GPU path."""
+    import numpy as np
+    import torch
+    prb, proj, pos, d, ref = sample
+    dev = lambda v: torch.from_numpy(np.ascontiguousarray(v)).cuda()  # noqa: E731
+    x = {"prb": dev(prb), "proj": dev(proj), "pos": dev(pos)}
+    out = {}
+    for path in ("fused", "cascade"):
+        g = {"prb": torch.zeros_like(x["prb"]), "proj": torch.empty_like(x["proj"]),
+             "pos": torch.empty_like(x["pos"])}
+        getattr(rec, f"gradients_{path}")(x, g, dev(d))
+        torch.cuda.synchronize()
+        errs = {}
+        for k, r in zip(("prb", "proj", "pos"), ref):
+            got = g[k].cpu().numpy().astype(np.complex128)
+            errs[k] = float(np.linalg.norm(got - r) / np.linalg.norm(r))
+        out[path if path == "fused" else "per_operator"] = errs
+    out["tolerance"] = "1e-5 (prb, proj), 1e-4 (pos)"
+    return out
 
 
 def _timed(fn, steps, warmup):
@@ -676,7 +708,8 @@ def main():
         "roofline": roof,
     }
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(a, host)
+        out["cpu_baseline"], sample = cpu_baseline(a, host, vars_, data)
+        out["parity_vs_fp64"] = parity_vs_fp64(rec, sample)
     print(json.dumps(out), flush=True)
 
 
