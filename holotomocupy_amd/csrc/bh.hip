@@ -41,18 +41,17 @@ struct BhArgs {
   const float2* taby = nullptr;      // [ndist][2][3*(nz/2)]
 };
 
-// 16-tap B-spline gather at sampling point x of a row whose y-taps start at iy
+// 16-tap B-spline gather at sampling position px of a row whose y-taps start at iy
 // with weights wy (and dwy).  v = sum wy wx c; with DERIV also
 // dt1 = -sum dwy wx c and dt2 = -sum wy dwx c (the dsadj weights,
 // cuda_kernels.py:735-748).
 template <typename TO, bool DERIV>
-__device__ __forceinline__ void s_gather(const TO* __restrict__ c, int npsi, int nzpsi, float x, int iy,
+__device__ __forceinline__ void s_gather(const TO* __restrict__ c, int npsi, int nzpsi, BsPos px, int iy,
                                          const float* wy, const float* dwy, TO& v, TO& dt1, TO& dt2) {
-  const float fx = floorf(x);
-  const int ix = (int)fx - 1;
+  const int ix = px.i - 1;
   float wx[4], dwx[4];
-  bs_w(x - fx, wx);
-  if (DERIV) bs_dw(x - fx, dwx);
+  bs_w(px.d, wx);
+  if (DERIV) bs_dw(px.d, dwx);
   v = vzero<TO>();
   dt1 = vzero<TO>();
   dt2 = vzero<TO>();
@@ -103,13 +102,12 @@ __device__ __forceinline__ void stage_row(const TO* __restrict__ cb, int npsi, i
 }
 
 template <typename TO, bool DERIV>
-__device__ __forceinline__ void row_gather(const TO* cy, const TO* cdy, int lo, int hi, float x, TO& v,
+__device__ __forceinline__ void row_gather(const TO* cy, const TO* cdy, int lo, int hi, BsPos px, TO& v,
                                            TO& dt1, TO& dt2) {
-  const float fx = floorf(x);
-  const int ix = (int)fx - 1;
+  const int ix = px.i - 1;
   float wx[4], dwx[4];
-  bs_w(x - fx, wx);
-  if (DERIV) bs_dw(x - fx, dwx);
+  bs_w(px.d, wx);
+  if (DERIV) bs_dw(px.d, dwx);
   v = vzero<TO>();
   dt1 = vzero<TO>();
   dt2 = vzero<TO>();
@@ -175,11 +173,10 @@ __global__ __launch_bounds__(kRowThreads) void bh_fwd_rows_k(BhArgs a) {
     asm volatile("" : "+v"(t));
     const int64_t bj = b * a.ndist + j;
     const float mb = a.mag[bj], ry = a.pos[2 * bj], rx = a.pos[2 * bj + 1];
-    const float y = bs_coord(ty, a.nz, a.nzpsi, mb, ry);
-    const float fy = floorf(y);
-    const int iy = (int)fy - 1;
+    const BsPos py = bs_pos(ty, a.nz, a.nzpsi, mb, ry);
+    const int iy = py.i - 1;
     float wy[4];
-    bs_w(y - fy, wy);
+    bs_w(py.d, wy);
     const TO* cb = (const TO*)a.c + b * plane;
     const int p = a.pidx ? a.pidx[b] : 0;
     const float2* prow = a.prb + (((int64_t)p * a.ndist + j) * a.nz + ty) * N;
@@ -193,7 +190,7 @@ __global__ __launch_bounds__(kRowThreads) void bh_fwd_rows_k(BhArgs a) {
       for (int e = 0; e < E; ++e) {
         const int i = t + e * T;
         TO v, d1, d2;
-        row_gather<TO, false>(cy, cy, lo, hi, bs_coord(i, N, a.npsi, mb, rx), v, d1, d2);
+        row_gather<TO, false>(cy, cy, lo, hi, bs_pos(i, N, a.npsi, mb, rx), v, d1, d2);
         s[row_slot<N, E>(t, e)] = cmul(prow[i], expi_fast(to_c(v)));
       }
     } else {
@@ -201,7 +198,7 @@ __global__ __launch_bounds__(kRowThreads) void bh_fwd_rows_k(BhArgs a) {
       for (int e = 0; e < E; ++e) {
         const int i = t + e * T;
         TO v, d1, d2;
-        s_gather<TO, false>(cb, a.npsi, a.nzpsi, bs_coord(i, N, a.npsi, mb, rx), iy, wy, wy, v, d1, d2);
+        s_gather<TO, false>(cb, a.npsi, a.nzpsi, bs_pos(i, N, a.npsi, mb, rx), iy, wy, wy, v, d1, d2);
         s[row_slot<N, E>(t, e)] = cmul(prow[i], expi_fast(to_c(v)));
       }
     }
@@ -330,12 +327,11 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
       s[row_slot<N, E>(t, e)] = valid ? row[i] : make_float2(0.f, 0.f);
     }
     const float mb = a.mag[bj], ry = a.pos[2 * bj], rx = a.pos[2 * bj + 1];
-    const float y = bs_coord(ty, a.nz, a.nzpsi, mb, ry);
-    const float fy = floorf(y);
-    const int iy = (int)fy - 1;
+    const BsPos py = bs_pos(ty, a.nz, a.nzpsi, mb, ry);
+    const int iy = py.i - 1;
     float wy[4], dwy[4];
-    bs_w(y - fy, wy);
-    bs_dw(y - fy, dwy);
+    bs_w(py.d, wy);
+    bs_dw(py.d, dwy);
     const TO* cb = (const TO*)a.c + b * plane;
     TO* cy = (TO*)dyn_lds() + 2 * r * a.npsi;
     TO* cdy = cy + a.npsi;
@@ -359,9 +355,9 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
       const float2 y12 = s[row_slot<N, E>(t, e)];
       TO v, d1, d2;
       if constexpr (STAGED)
-        row_gather<TO, true>(cy, cdy, lo, hi, bs_coord(i, N, a.npsi, mb, rx), v, d1, d2);
+        row_gather<TO, true>(cy, cdy, lo, hi, bs_pos(i, N, a.npsi, mb, rx), v, d1, d2);
       else
-        s_gather<TO, true>(cb, a.npsi, a.nzpsi, bs_coord(i, N, a.npsi, mb, rx), iy, wy, dwy, v, d1, d2);
+        s_gather<TO, true>(cb, a.npsi, a.nzpsi, bs_pos(i, N, a.npsi, mb, rx), iy, wy, dwy, v, d1, d2);
       const float2 ex = expi_fast(to_c(v));
       acc[i] = cadd(acc[i], cmulc(y12, ex));                // y12 conj(x12)
       const float2 q = cmulc(cmulc(y12, prow[i]), ex);      // y12 conj(prb) conj(x12)
@@ -419,7 +415,7 @@ __global__ __launch_bounds__(256) void bh_sadj_k(BhArgs a, TO* __restrict__ dpsi
         int ylo = 0, yhi = -1;
         if (iy < a.nzpsi) {
           bs_candidates(iy, a.nz, a.nzpsi, mb, ry, ylo, yhi);
-          if (ylo + k <= yhi) w = bs_reach(bs_coord(ylo + k, a.nz, a.nzpsi, mb, ry), iy);
+          if (ylo + k <= yhi) w = bs_reach(bs_pos(ylo + k, a.nz, a.nzpsi, mb, ry), iy);
         }
         s_wy[row][k] = w;
         if (k == 0) s_ylo[row] = ylo;
@@ -431,7 +427,7 @@ __global__ __launch_bounds__(256) void bh_sadj_k(BhArgs a, TO* __restrict__ dpsi
         float wxs[K];
 #pragma unroll
         for (int k = 0; k < K; ++k)
-          wxs[k] = xlo + k <= xhi ? bs_reach(bs_coord(xlo + k, a.n, a.npsi, mb, rx), ix) : 0.0f;
+          wxs[k] = xlo + k <= xhi ? bs_reach(bs_pos(xlo + k, a.n, a.npsi, mb, rx), ix) : 0.0f;
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
           const int ylo = s_ylo[r0 + rr];
@@ -457,11 +453,11 @@ __global__ __launch_bounds__(256) void bh_sadj_k(BhArgs a, TO* __restrict__ dpsi
         int ylo, yhi;
         bs_candidates(iy, a.nz, a.nzpsi, mb, ry, ylo, yhi);
         for (int ty = ylo; ty <= yhi; ++ty) {
-          const float wyv = bs_reach(bs_coord(ty, a.nz, a.nzpsi, mb, ry), iy);
+          const float wyv = bs_reach(bs_pos(ty, a.nz, a.nzpsi, mb, ry), iy);
           if (wyv == 0.0f) continue;
           TO rowv = vzero<TO>();
           for (int tx = xlo; tx <= xhi; ++tx) {
-            const float wxv = bs_reach(bs_coord(tx, a.n, a.npsi, mb, rx), ix);
+            const float wxv = bs_reach(bs_pos(tx, a.n, a.npsi, mb, rx), ix);
             if (wxv != 0.0f) rowv = vfma(wxv, from_c<TO>(g[(int64_t)ty * a.n + tx]), rowv);
           }
           acc[rr] = vfma(wyv, rowv, acc[rr]);

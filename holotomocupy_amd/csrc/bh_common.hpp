@@ -27,11 +27,12 @@ __device__ __forceinline__ unsigned char* dyn_lds() {
 
 template <int N>
 __device__ __forceinline__ void row_span(int npsi, float mb, float rx, int& lo, int& hi) {
-  // bs_coord is monotone in the pixel index, so the row's taps lie within
-  // [floor(x(0)) - 1, floor(x(N-1)) + 2]
-  const float x0 = bs_coord(0, N, npsi, mb, rx), x1 = bs_coord(N - 1, N, npsi, mb, rx);
-  lo = max((int)floorf(fminf(x0, x1)) - 1, 0);
-  hi = min((int)floorf(fmaxf(x0, x1)) + 3, npsi);
+  // the sampling position is monotone in the pixel index, so the row's taps
+  // lie within [floor(x(0)) - 1, floor(x(N-1)) + 2] (the floors of bs_pos, which
+  // every gather uses)
+  const int i0 = bs_pos(0, N, npsi, mb, rx).i, i1 = bs_pos(N - 1, N, npsi, mb, rx).i;
+  lo = max(min(i0, i1) - 1, 0);
+  hi = min(max(i0, i1) + 3, npsi);
 }
 
 // Optional timing hook: events[2k] / events[2k+1] are recorded before / after

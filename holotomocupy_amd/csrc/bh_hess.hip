@@ -106,13 +106,12 @@ __global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
     const float yyx = a.dy[2 * bj + 1], yyy = a.dy[2 * bj];
     const float zzx = SAME ? yyx : a.dz[2 * bj + 1], zzy = SAME ? yyy : a.dz[2 * bj];
     const float cross = yyx * zzy + yyy * zzx;
-    const float y = bs_coord(ty, a.nz, a.nzpsi, mb, ry);
-    const float fy = floorf(y);
-    const int iy = (int)fy - 1;
+    const BsPos py = bs_pos(ty, a.nz, a.nzpsi, mb, ry);
+    const int iy = py.i - 1;
     float wy[4], dwy[4], d2wy[4];
-    bs_w(y - fy, wy);
-    bs_dw(y - fy, dwy);
-    bs_d2w(y - fy, d2wy);
+    bs_w(py.d, wy);
+    bs_dw(py.d, dwy);
+    bs_d2w(py.d, d2wy);
     const TO* cb = (const TO*)a.c + b * plane;
     const TO* cyb = (const TO*)a.cy + b * plane;
     const TO* czb = (const TO*)a.cz + b * plane;
@@ -158,13 +157,12 @@ __global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
     float2 ev[E], y12[E], z12[SAME ? 1 : E], w12[E], v12[PAIR ? E : 1];
     auto chain = [&](int e) {
       const int i = t + e * T;
-      const float x = bs_coord(i, N, a.npsi, mb, rx);
-      const float fx = floorf(x);
-      const int ix = (int)fx - 1;
+      const BsPos px = bs_pos(i, N, a.npsi, mb, rx);
+      const int ix = px.i - 1;
       float wx[4], dwx[4], d2wx[4];
-      bs_w(x - fx, wx);
-      bs_dw(x - fx, dwx);
-      bs_d2w(x - fx, d2wx);
+      bs_w(px.d, wx);
+      bs_dw(px.d, dwx);
+      bs_d2w(px.d, d2wx);
       TO x22 = vzero<TO>(), sx = vzero<TO>(), sxx = vzero<TO>(), ay = vzero<TO>(), az = vzero<TO>();
       TO w1 = vzero<TO>(), w0 = vzero<TO>(), v1 = vzero<TO>(), v0 = vzero<TO>();
 #pragma unroll

@@ -32,13 +32,13 @@ __global__ __launch_bounds__(kTX* kTY) void bs_forward_k(BsArgs<T> a) {
   const int ty = tyt * kTY + (int)(threadIdx.x / kTX);
   if (tx >= a.n || ty >= a.nz) return;
   const float mb = a.m[b * a.ms];
-  const float x = bs_coord(tx, a.n, a.npsi, mb, a.r[b * a.rs + 1]);
-  const float y = bs_coord(ty, a.nz, a.nzpsi, mb, a.r[b * a.rs + 0]);
-  const float fx = floorf(x), fy = floorf(y);
-  const int ix = (int)fx - 1, iy = (int)fy - 1;
+  const BsPos px = bs_pos(tx, a.n, a.npsi, mb, a.r[b * a.rs + 1]);
+  const BsPos py = bs_pos(ty, a.nz, a.nzpsi, mb, a.r[b * a.rs + 0]);
+  const int ix = px.i - 1, iy = py.i - 1;
+  const float xd = px.d, yd = py.d;
   float wx[4], wy[4];
-  bs_w(x - fx, wx);
-  bs_w(y - fy, wy);
+  bs_w(xd, wx);
+  bs_w(yd, wy);
   const int64_t plane = (int64_t)a.nzpsi * a.npsi;
   const T* c = a.c + b * plane;
   T acc = vzero<T>();
@@ -58,8 +58,8 @@ __global__ __launch_bounds__(kTX* kTY) void bs_forward_k(BsArgs<T> a) {
     }
   } else {
     float dwx[4], dwy[4];
-    bs_dw(x - fx, dwx);
-    bs_dw(y - fy, dwy);
+    bs_dw(xd, dwx);
+    bs_dw(yd, dwy);
     const T* c1 = a.c1 + b * plane;
     if constexpr (MODE == BS_D1) {
       const float drx = a.dr1[b * a.d1s + 1], dry = a.dr1[b * a.d1s + 0];
@@ -79,8 +79,8 @@ __global__ __launch_bounds__(kTX* kTY) void bs_forward_k(BsArgs<T> a) {
       }
     } else {
       float d2wx[4], d2wy[4];
-      bs_d2w(x - fx, d2wx);
-      bs_d2w(y - fy, d2wy);
+      bs_d2w(xd, d2wx);
+      bs_d2w(yd, d2wy);
       const T* c2 = a.c2 + b * plane;
       const float d1x = a.dr1[b * a.d1s + 1], d1y = a.dr1[b * a.d1s + 0];
       const float d2x = a.dr2[b * a.d2s + 1], d2y = a.dr2[b * a.d2s + 0];
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void bs_adjoint_k(BsArgs<T> a) {
       int ylo = 0, yhi = -1;
       if (iy < a.nzpsi) {
         bs_candidates(iy, a.nz, a.nzpsi, mb, ry, ylo, yhi);
-        if (ylo + k <= yhi) w = bs_reach(bs_coord(ylo + k, a.nz, a.nzpsi, mb, ry), iy);
+        if (ylo + k <= yhi) w = bs_reach(bs_pos(ylo + k, a.nz, a.nzpsi, mb, ry), iy);
       }
       s_wy[row][k] = w;
       if (k == 0) s_ylo[row] = ylo;
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void bs_adjoint_k(BsArgs<T> a) {
       float wxs[K];
 #pragma unroll
       for (int k = 0; k < K; ++k)
-        wxs[k] = xlo + k <= xhi ? bs_reach(bs_coord(xlo + k, a.n, a.npsi, mb, rx), ix) : 0.0f;
+        wxs[k] = xlo + k <= xhi ? bs_reach(bs_pos(xlo + k, a.n, a.npsi, mb, rx), ix) : 0.0f;
 #pragma unroll
       for (int rr = 0; rr < R; ++rr) {
         const int ylo = s_ylo[r0 + rr];
@@ -188,11 +188,11 @@ __global__ __launch_bounds__(256) void bs_adjoint_k(BsArgs<T> a) {
       int ylo, yhi;
       bs_candidates(iy, a.nz, a.nzpsi, mb, ry, ylo, yhi);
       for (int ty = ylo; ty <= yhi; ++ty) {
-        const float wyv = bs_reach(bs_coord(ty, a.nz, a.nzpsi, mb, ry), iy);
+        const float wyv = bs_reach(bs_pos(ty, a.nz, a.nzpsi, mb, ry), iy);
         if (wyv == 0.0f) continue;
         T row = vzero<T>();
         for (int tx = xlo; tx <= xhi; ++tx) {
-          const float wxv = bs_reach(bs_coord(tx, a.n, a.npsi, mb, rx), ix);
+          const float wxv = bs_reach(bs_pos(tx, a.n, a.npsi, mb, rx), ix);
           if (wxv != 0.0f) row = vfma(wxv, g[(int64_t)ty * a.n + tx], row);
         }
         acc[rr] = vfma(wyv, row, acc[rr]);
@@ -224,15 +224,14 @@ __global__ __launch_bounds__(kTX* kTY) void bs_dadj_partial_k(BsArgs<T> a, const
   float s1 = 0.0f, s2 = 0.0f;
   if (tx < a.n && ty < a.nz) {
     const float mb = a.m[b * a.ms];
-    const float x = bs_coord(tx, a.n, a.npsi, mb, a.r[b * a.rs + 1]);
-    const float y = bs_coord(ty, a.nz, a.nzpsi, mb, a.r[b * a.rs + 0]);
-    const float fx = floorf(x), fy = floorf(y);
-    const int ix = (int)fx - 1, iy = (int)fy - 1;
+    const BsPos px = bs_pos(tx, a.n, a.npsi, mb, a.r[b * a.rs + 1]);
+    const BsPos py = bs_pos(ty, a.nz, a.nzpsi, mb, a.r[b * a.rs + 0]);
+    const int ix = px.i - 1, iy = py.i - 1;
     float wx[4], wy[4], dwx[4], dwy[4];
-    bs_w(x - fx, wx);
-    bs_w(y - fy, wy);
-    bs_dw(x - fx, dwx);
-    bs_dw(y - fy, dwy);
+    bs_w(px.d, wx);
+    bs_w(py.d, wy);
+    bs_dw(px.d, dwx);
+    bs_dw(py.d, dwy);
     const T* c = a.c + b * (int64_t)a.nzpsi * a.npsi;
     T dt1 = vzero<T>(), dt2 = vzero<T>();
 #pragma unroll

@@ -43,11 +43,30 @@ __device__ __forceinline__ void bs_d2w(float d, float* w) {
   w[3] = 6.0f * d;
 }
 
-// Sampling coordinate of detector index t (cuda_kernels.py:174-177), evaluated
-// with the reference's operation order.
+// Sampling coordinate of detector index t (cuda_kernels.py:174-177), in FP32
+// with the reference's operation order: used only to bracket candidate taps
+// (every caller keeps a one-pixel margin).
 __device__ __forceinline__ float bs_coord(int t, int npix, int npsi, float m, float r) {
   const float half = (m - 1.0f) / 2.0f;
   return (m * (float)(t - npix / 2) - r + half) + (float)(npsi / 2);
+}
+
+// The same coordinate split into its integer part and fractional offset, which
+// is what the taps use: x = m (t - npix/2) - r + (m - 1)/2 + npsi/2 is formed in
+// FP64 (exact for these operands), so the offset d in [0, 1) carries FP32
+// precision.  An FP32 coordinate near 1000 is only good to 6e-5 of a pixel,
+// the dominant error of S against FP64 at 2048^2 detectors.  Every tap weight
+// (S, its derivatives and the gather adjoints) comes from this one function, so
+// the adjoint pairs use identical weights.
+struct BsPos {
+  int i;    // floor(x)
+  float d;  // x - floor(x)
+};
+__device__ __forceinline__ BsPos bs_pos(int t, int npix, int npsi, float m, float r) {
+  const float half = (m - 1.0f) / 2.0f;
+  const double x = (double)m * (double)(t - npix / 2) - (double)r + (double)half + (double)(npsi / 2);
+  const double fx = floor(x);
+  return BsPos{(int)fx, (float)(x - fx)};
 }
 
 // Sadj as a gather: candidate detector pixels [lo, hi] whose taps can reach
@@ -64,13 +83,12 @@ __device__ __forceinline__ void bs_candidates(int i, int npix, int npsi, float m
   hi = hi > npix - 1 ? npix - 1 : hi;
 }
 
-// weight with which pixel coordinate `x` reaches coefficient index i (0 if none)
-__device__ __forceinline__ float bs_reach(float x, int i) {
-  const float fx = floorf(x);
-  const int j = i - (int)fx;  // tap offset, valid in [-1, 2]
+// weight with which sampling position p reaches coefficient index i (0 if none)
+__device__ __forceinline__ float bs_reach(BsPos p, int i) {
+  const int j = i - p.i;  // tap offset, valid in [-1, 2]
   if (j < -1 || j > 2) return 0.0f;
   float w[4];
-  bs_w(x - fx, w);
+  bs_w(p.d, w);
   return w[j + 1];
 }
 
