@@ -239,8 +239,7 @@ def cpu_baseline(a, host, vars_, data):
     from oracle import shift as osh
     from holotomocupy_amd.rec import physics
     _, _, _, args, g = host
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    cores = max(1, min(cores, os.cpu_count() or 1))
+    cores, model = _host_cpu()
     op.WORKERS = cores
     osh.WORKERS = cores
     ph = physics(g["energy"], g["z1"], g["focustodetectordistance"], g["detector_pixelsize"])
@@ -257,7 +256,7 @@ def cpu_baseline(a, host, vars_, data):
     t0 = time.perf_counter()
     ref = cas.gradients(*x, d.astype(np.float64))
     dt = time.perf_counter() - t0
-    base = {"value": round(1.0 / dt, 5), "unit": "evals/s", "cores": cores, "kind": "port",
+    base = {"value": round(1.0 / dt, 5), "unit": "evals/s", "cores": cores, "cpu_model": model, "kind": "port",
             "sample": f"1 angle x {a.ndist} distances at {a.n}^2 / object {a.nobj}^2, FP64 NumPy/SciPy "
                       f"oracle cascade (scipy.fft workers={cores}), one run of {dt:.1f} s"}
     return base, (prb, proj, pos, d, ref)
@@ -285,6 +284,37 @@ def parity_vs_fp64(rec, sample):
         out[path if path == "fused" else "per_operator"] = errs
     out["tolerance"] = "1e-5 (prb, proj), 1e-4 (pos)"
     return out
+
+
+def _host_cpu():
+    """(threads used by the oracle, CPU model) of this host."""
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cores = max(1, min(cores, os.cpu_count() or 1))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, model
+
+
+def _op_cpu_baseline(fn, evals, unit, sample):
+    """Time fn() (the oracle on a bounded sample worth `evals` units) once
+    after one warm-up call."""
+    from oracle import propagation as op
+    from oracle import shift as osh
+    cores, model = _host_cpu()
+    op.WORKERS = osh.WORKERS = cores
+    fn()
+    t0 = time.perf_counter()
+    fn()
+    dt = time.perf_counter() - t0
+    return {"value": round(evals / dt, 5), "unit": unit, "cores": cores, "cpu_model": model, "kind": "port",
+            "sample": f"{sample}; FP64 NumPy/SciPy oracle, one run of {dt:.2f} s"}
 
 
 def _timed(fn, steps, warmup):
@@ -336,8 +366,20 @@ def op_prop(a):
     ms = _timed(step, a.steps, a.warmup)
     F = 8 * n * n
     nbytes = nd * 2 * B * 4 * F
-    return _op_line("D+DT field evals/s (16 x 256^2, 4 distances)", nd * 2 * B / (ms * 1e-3), "evals/s", ms,
+    line = _op_line("D+DT field evals/s (16 x 256^2, 4 distances)", nd * 2 * B / (ms * 1e-3), "evals/s", ms,
                     nbytes, "Fresnel D then DT, 16 fields 256^2, 4 distances", a, {"step": ms})
+    if not a.no_cpu_baseline:
+        from oracle import propagation as op
+        o = op.Propagation(n, n, B, nd, 1.0, 1.0, 1.0 / np.array([1e-3, 1.2e-3, 1.5e-3, 2e-3]))
+        xs = x.cpu().numpy().astype(np.complex128)
+
+        def oracle_step():
+            for j in range(nd):
+                o.DT(o.D(xs, j), j)
+
+        line["cpu_baseline"] = _op_cpu_baseline(oracle_step, nd * 2 * B, "evals/s",
+                                                "the 16 fields, D then DT at 4 distances")
+    return line
 
 
 def op_tomo(a):
@@ -355,8 +397,17 @@ def op_tomo(a):
     ms_rt = _timed(lambda: tomo.RT(d), a.steps, a.warmup)
     nbytes = 2 * n * 8 * (13 * n * n + nth * n)
     ms = ms_r + ms_rt
-    return _op_line("R+RT evals/s (256^3, 384 angles)", 1.0 / (ms * 1e-3), "evals/s", ms, nbytes,
+    line = _op_line("R+RT evals/s (256^3, 384 angles)", 1.0 / (ms * 1e-3), "evals/s", ms, nbytes,
                     "USFFT Radon R and RT, 256 slices of 256^2, 384 angles", a, {"R": ms_r, "RT": ms_rt})
+    if not a.no_cpu_baseline:
+        from oracle import tomo as ot
+        ns = 4
+        o = ot.Tomo(n, ns, np.arange(nth) * np.pi / nth, 0)
+        us = u[:ns].cpu().numpy().astype(np.complex128)
+        ds = d[:, :ns].cpu().numpy().astype(np.complex128)
+        line["cpu_baseline"] = _op_cpu_baseline(lambda: (o.R(us), o.RT(ds)), ns / n, "evals/s",
+                                                f"{ns} of the {n} slices (R and RT), scaled to the volume")
+    return line
 
 
 def op_shift(a):
@@ -379,9 +430,23 @@ def op_shift(a):
     F = 8 * n * n
     nbytes = B * (2 * F + 2 * F + 2 * F)
     ms = ms_c + ms_s + ms_a
-    return _op_line("coeff+S+Sadj plane evals/s (384 x 256^2)", B / (ms * 1e-3), "evals/s", ms, nbytes,
+    line = _op_line("coeff+S+Sadj plane evals/s (384 x 256^2)", B / (ms * 1e-3), "evals/s", ms, nbytes,
                     "B-spline prefilter, shift and adjoint shift, 384 planes of 256^2", a,
                     {"coeff": ms_c, "S": ms_s, "Sadj": ms_a})
+    if not a.no_cpu_baseline:
+        from oracle import shift as osh
+        ns = 128
+        o = osh.Shift(n, n, n, n, "complex64")
+        ps = psi[:ns].cpu().numpy().astype(np.complex128)
+        rs, ms_ = r[:ns].cpu().numpy(), m[:ns].cpu().numpy()
+
+        def oracle_step():
+            cs = o.coeff(ps)
+            o.S(cs, rs, ms_)
+            o.Sadj(ps, rs, ms_)
+
+        line["cpu_baseline"] = _op_cpu_baseline(oracle_step, ns, "evals/s", f"{ns} of the {B} planes")
+    return line
 
 
 def op_bh(a):
