@@ -286,6 +286,25 @@ def parity_vs_fp64(rec, sample):
     return out
 
 
+def measured_copy_gbs(nbytes=1 << 30, reps=10):
+    """Streaming device-to-device copy on this GPU: (bytes read + bytes
+    written) / time, mean over `reps` copies of nbytes after two warm-ups."""
+    import torch
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return round(2 * nbytes / (ms * 1e-3) / 1e9, 1)
+
+
 def _host_cpu():
     """(threads used by the oracle, CPU model) of this host."""
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -772,6 +791,8 @@ def main():
         "speedup_vs_plain": round(value / plain_value, 3) if plain_value else None,
         "roofline": roof,
     }
+    if isinstance(roof, dict):
+        roof["measured_copy_gbs"] = measured_copy_gbs()  # beside the 8 TB/s spec peak
     if not a.no_cpu_baseline:
         out["cpu_baseline"], sample = cpu_baseline(a, host, vars_, data)
         out["parity_vs_fp64"] = parity_vs_fp64(rec, sample)
