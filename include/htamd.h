@@ -34,6 +34,9 @@ extern "C" {
 
 typedef struct htamd_prop_plan htamd_prop_plan;
 
+/* Library housekeeping, no counterpart in the reference (which raises Python
+ * exceptions from CuPy): the message of this thread's last failed call, and
+ * the ABI version the loader checks. */
 const char* htamd_last_error(void);
 int htamd_abi_version(void);
 
@@ -120,7 +123,9 @@ int htamd_bspline_d2(const htamd_shift_plan* plan, const void* c, const void* c1
                      const float* dr1, int64_t dr1_stride, const float* dr2, int64_t dr2_stride,
                      void* out, int64_t batch, void* stream);
 
-/* Bytes of device workspace htamd_bspline_d1_adj needs for `batch` angles. */
+/* Bytes of device workspace htamd_bspline_d1_adj needs for `batch` angles
+ * (the per-workgroup partial sums of dcurlySadjc's out2, shift.py:185-204; the
+ * reference allocates its temporaries inside the call). */
 int htamd_bspline_d1_adj_workspace_size(const htamd_shift_plan* plan, int64_t batch, int64_t* bytes);
 
 /* Adjoint of the differential: out1[batch, nzpsi, npsi] = Sadj(g), and
