@@ -106,13 +106,13 @@ def served_run(a, rec, vars_, data, nreq, warmup):
     srv = BHServer(max_batch_angles=max(a.angles, 64))
     mk = lambda: GradientRequest(geo, vars_["prb"], vars_["proj"], vars_["pos"], data,  # noqa: E731
                                  data_size=rec.data_size, rho=(1.0, 0.05, 0.02))
-    for _ in range(max(2, warmup)):
-        for f in [srv.submit(mk()) for _ in range(nreq)]:
+    for _ in range(max(2, warmup)):  # a burst is submitted atomically, so its batches repeat
+        for f in srv.submit_many([mk() for _ in range(nreq)]):
             f.result()
     torch.cuda.synchronize()
     reqs = [mk() for _ in range(nreq)]
     t0 = time.perf_counter()
-    futs = [srv.submit(r) for r in reqs]
+    futs = srv.submit_many(reqs)
     for f in futs:
         f.result()
     wall = time.perf_counter() - t0

@@ -571,6 +571,14 @@ class BHServer:
             self._lock.notify()
         return fut
 
+    def submit_many(self, reqs):
+        """Submit a group atomically: the loop sees all of them at once, so a
+        group's batches do not depend on thread timing.  Returns the futures."""
+        with self._lock:
+            futs = [self.sched.submit(r) for r in reqs]
+            self._lock.notify()
+        return futs
+
     def _loop(self):
         while True:
             with self._lock:

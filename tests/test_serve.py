@@ -282,3 +282,18 @@ def test_tomo_requests_equal_direct_calls(gpu):
         assert torch.equal(got, t.R(o))
     for d, got in zip(sinos, res[2:]):
         assert torch.equal(got, t.RT(d))
+
+
+def test_submit_many_forms_the_same_batches():
+    ex = FakeExec(per_angle=1)
+    srv = BHServer(hbm_budget=10 ** 9, executor=ex)
+    for _ in range(3):
+        futs = srv.submit_many([req(G1, 16) for _ in range(10)])
+        ex.pending_done.update(range(1000))
+        for f in futs:
+            f.result(timeout=5)
+    srv.close()
+    # every burst: the scheduler sees all 10 at once and packs them the same way
+    sizes = [b for _, b in ex.runs]
+    per_burst = len(sizes) // 3
+    assert sizes[:per_burst] == sizes[per_burst:2 * per_burst] == sizes[2 * per_burst:]
