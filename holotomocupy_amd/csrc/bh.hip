@@ -534,7 +534,8 @@ constexpr int bh_col_e() {
 // threads and the 160 KiB of LDS of a CU.  The pass does two operators per
 // load and store, so the strided access matters less than for a single
 // operator; at 2048 (16 elements per thread) C = 2 and C = 4 measure the
-// same, 32 elements with C = 8 and 8 elements with C = 4 are slower.
+// same, 32 elements with C = 8 and 8 elements with C = 4 are slower, and
+// C = 1 (8-byte rows) takes 5.5 ms against 3.6.
 template <int NZ, int C>
 constexpr bool col_tile_fits() {
   return C * (NZ / bh_col_e<NZ>()) <= 256 && (int64_t)C * col_stride<NZ, C>() * 8 + 4096 <= 160 * 1024;
