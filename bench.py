@@ -36,11 +36,12 @@ def parse():
     p.add_argument("--path", choices=["auto", "plain", "fused", "served"], default="auto")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison run")
-    p.add_argument("--op", choices=["prop", "tomo", "shift", "bh", "serve-mix"], default=None,
+    p.add_argument("--op", choices=["prop", "tomo", "shift", "bh", "serve-mix", "chunking"], default=None,
                    help="operator benchmark instead of the BH gradient: prop = D then DT, 16 fields of "
                         "256^2 at 4 distances; tomo = R and RT at 256^3, 384 angles; shift = coeff, S and "
                         "Sadj at 384 x 256^2; bh = Rec.BH iterations at 512^3, 720 angles, 1024^2; serve-mix = "
-                        "Poisson mix of operator, gradient and bilinear-form requests (n 256..2048)")
+                        "Poisson mix of operator, gradient and bilinear-form requests (n 256..2048); chunking = R "
+                        "and RT through gpu_batch from host memory at 256^3, 384 angles")
     p.add_argument("--bh-config", choices=["c3", "c4"], default="c3",
                    help="--op bh sizes: c3 = object 512^3, 720 angles, 1024^2 detector; c4 = object 1024^3, "
                         "1800 angles, 2048^2 detector (4 distances both)")
@@ -468,6 +469,52 @@ def op_shift(a):
     return line
 
 
+def op_chunking(a):
+    """Rec.fwd_tomo (R) and Rec.gF4 (RT) over a host-resident 256^3 object and
+    384-angle sinogram through Chunking.gpu_batch (chunks of 16 slices, pinned
+    double-buffered staging, copies and compute on separate streams ordered
+    by events), beside the same operators on device-resident arrays.  Bytes:
+    the host <-> device traffic of one pass (object in + sinogram out)."""
+    import numpy as np
+    import torch
+    from types import SimpleNamespace
+    from holotomocupy_amd.rec import Rec
+    n, nth = 256, 384
+    theta = np.arange(nth) * np.pi / nth
+    args = SimpleNamespace(ndist=1, ntheta=nth, nz=n, n=n, nzobj=n, nobj=n, obj_dtype="complex64",
+                           rho=[1.0, 1.0, 1.0], nchunk=16, theta=theta, mask_r=0, **geometry(n, 1))
+    rec = Rec(args)
+    rng = np.random.default_rng(1002)
+    u = (rng.standard_normal((n, n, n)) + 1j * rng.standard_normal((n, n, n))).astype(np.complex64)
+    d = (rng.standard_normal((nth, n, n)) + 1j * rng.standard_normal((nth, n, n))).astype(np.complex64)
+    out_h, gu_h = np.empty((nth, n, n), np.complex64), np.empty((n, n, n), np.complex64)
+    u_d, d_d = torch.from_numpy(u).cuda(), torch.from_numpy(d).cuda()
+    out_d, gu_d = torch.empty((nth, n, n), dtype=torch.complex64, device="cuda"), torch.empty_like(u_d)
+
+    def wall(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    reps = max(3, a.steps // 2)
+    parts = {"R_host": wall(lambda: rec.fwd_tomo(u, out_h), reps),
+             "RT_host": wall(lambda: rec.gF4(gu_h, d), reps),
+             "R_device": wall(lambda: rec.fwd_tomo(u_d, out_d), reps),
+             "RT_device": wall(lambda: rec.gF4(gu_d, d_d), reps)}
+    ms = parts["R_host"] + parts["RT_host"]
+    nbytes = 2 * (u.nbytes + d.nbytes)  # each operator moves the object and the sinogram once
+    line = _op_line("R+RT evals/s through gpu_batch from host memory (256^3, 384 angles)", 1e3 / ms, "evals/s", ms,
+                    nbytes, "Rec.fwd_tomo and Rec.gF4 on host-resident arrays, chunks of 16 slices", a, parts)
+    line["roofline"] = {"bound": "pcie", "host_device_gbs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                        "bytes_per_step": int(nbytes), "device_resident_ms": round(parts["R_device"]
+                                                                                   + parts["RT_device"], 3)}
+    return line
+
+
 def op_bh(a):
     """Rec.BH iterations on one GPU (16-angle launches, 16-slice tomography
     chunks, Laplacian regulariser on): `--bh-config c3` is object 512^3, 720
@@ -746,7 +793,7 @@ def main():
     torch.cuda.set_device(0)
     if a.op:
         print(json.dumps({"prop": op_prop, "tomo": op_tomo, "shift": op_shift, "bh": op_bh,
-                          "serve-mix": op_serve_mix}[a.op](a)), flush=True)
+                          "serve-mix": op_serve_mix, "chunking": op_chunking}[a.op](a)), flush=True)
         return
     rec, vars_, data, host = make_inputs(a)
     grads = {"prb": torch.zeros_like(vars_["prb"]), "proj": torch.empty_like(vars_["proj"]),

@@ -411,9 +411,21 @@ class Rec(BHLoop):
         needs args.theta (and args.mask_r, default 0)."""
         if getattr(self, "cl_tomo", None) is None:
             from .tomo import Tomo
-            self.cl_tomo = Tomo(self.nobj, getattr(self, "nchunk", self.nzobj), self.theta,
+            # the plan does not depend on the slice count: accept whole device slabs
+            self.cl_tomo = Tomo(self.nobj, max(getattr(self, "nchunk", self.nzobj), self.nzobj), self.theta,
                                 getattr(self, "mask_r", 0))
         return self.cl_tomo
+
+    def _tomo_slab(self):
+        """Slices per R / RT call on device-resident arrays: as many as keep the
+        oversampled grid workspace (32 nobj^2 bytes per slice) within 4 GiB, a
+        multiple of 32 (the RT gather's slice block) when that many fit, and at
+        least nchunk."""
+        per_slice = 32 * self.nobj * self.nobj
+        slab = max(1, (4 << 30) // per_slice)
+        if slab >= 32:
+            slab -= slab % 32
+        return int(min(self.nzobj, max(slab, getattr(self, "nchunk", 1))))
 
     def _chunking(self):
         """Chunking pool of rec_mpi.py:39-42 (16 nchunk nobj max(nobj, nzobj, ntheta) x 8 bytes)."""
@@ -430,8 +442,9 @@ class Rec(BHLoop):
         cl = self._chunking()
         tomo = self._tomo()
         if isinstance(obj, torch.Tensor) and obj.is_cuda and isinstance(out, torch.Tensor) and out.is_cuda:
-            for z0 in range(0, obj.shape[0], tomo._nz):
-                z1 = min(obj.shape[0], z0 + tomo._nz)
+            slab = self._tomo_slab()
+            for z0 in range(0, obj.shape[0], slab):
+                z1 = min(obj.shape[0], z0 + slab)
                 out[:, z0:z1] = tomo.R(obj[z0:z1])
             return out
 
@@ -449,9 +462,11 @@ class Rec(BHLoop):
         tomo = self._tomo()
         if isinstance(gradu, torch.Tensor) and gradu.is_cuda and isinstance(gradproj, torch.Tensor) \
                 and gradproj.is_cuda:
-            for z0 in range(0, gradu.shape[0], tomo._nz):
-                z1 = min(gradu.shape[0], z0 + tomo._nz)
-                gradu[z0:z1] = tomo.RT(gradproj[:, z0:z1].contiguous())
+            slab = self._tomo_slab()
+            for z0 in range(0, gradu.shape[0], slab):
+                z1 = min(gradu.shape[0], z0 + slab)
+                gradu[z0:z1] = tomo.RT(gradproj if (z0, z1) == (0, gradu.shape[0]) else
+                                       gradproj[:, z0:z1].contiguous())
             return
 
         @cl.gpu_batch(axis_out=0, axis_inp=1, nout=1)
