@@ -443,6 +443,8 @@ class Rec(BHLoop):
         tomo = self._tomo()
         if isinstance(obj, torch.Tensor) and obj.is_cuda and isinstance(out, torch.Tensor) and out.is_cuda:
             slab = self._tomo_slab()
+            if slab >= obj.shape[0] and out.is_contiguous() and out.dtype == obj.dtype:
+                return tomo.R(obj, out=out)  # one slab: straight into out
             for z0 in range(0, obj.shape[0], slab):
                 z1 = min(obj.shape[0], z0 + slab)
                 out[:, z0:z1] = tomo.R(obj[z0:z1])
@@ -465,8 +467,12 @@ class Rec(BHLoop):
             slab = self._tomo_slab()
             for z0 in range(0, gradu.shape[0], slab):
                 z1 = min(gradu.shape[0], z0 + slab)
-                gradu[z0:z1] = tomo.RT(gradproj if (z0, z1) == (0, gradu.shape[0]) else
-                                       gradproj[:, z0:z1].contiguous())
+                src = gradproj if (z0, z1) == (0, gradu.shape[0]) else gradproj[:, z0:z1].contiguous()
+                dst = gradu[z0:z1]  # whole slices: contiguous when gradu is
+                if dst.is_contiguous() and dst.dtype == src.dtype:
+                    tomo.RT(src, out=dst)
+                else:
+                    dst[:] = tomo.RT(src)
             return
 
         @cl.gpu_batch(axis_out=0, axis_inp=1, nout=1)

@@ -73,30 +73,36 @@ class Tomo:
         _lib.call("htamd_tomo_workspace_size", self._plan, nz, ctypes.byref(nbytes))
         return torch.empty(nbytes.value, dtype=torch.uint8, device=self.device)
 
-    def _run(self, name, x, out_shape):
+    def _run(self, name, x, out_shape, out=None):
         is_complex = x.is_complex()
         x = _lib.as_device(x, torch.complex64 if is_complex else torch.float32, self.device)
         nz = out_shape[0] if name == "htamd_tomo_RT" else out_shape[1]
         if nz > self._nz:
             raise ValueError(f"{nz} slices exceed the {self._nz} this Tomo was built for")
-        out = torch.empty(out_shape, dtype=x.dtype, device=self.device)
+        if out is None:
+            out = torch.empty(out_shape, dtype=x.dtype, device=self.device)
+        elif (tuple(out.shape) != tuple(out_shape) or out.dtype != x.dtype or not out.is_contiguous()
+              or out.device != self.device):
+            raise ValueError(f"out must be a contiguous {x.dtype} tensor of shape {tuple(out_shape)} on {self.device}")
         ws = self._workspace(nz)
         _lib.call(name, self._plan, _lib.ptr(x), 1 if is_complex else 0, _lib.ptr(out), nz, _lib.ptr(ws),
                   ws.numel(), _lib.stream_ptr())
         return out
 
-    def R(self, obj):
+    def R(self, obj, out=None):
         """Radon transform (tomo.py:56-86): obj [nz', n, n] -> [ntheta, nz', n]
-        (float32 in -> float32 out, complex64 -> complex64)."""
+        (float32 in -> float32 out, complex64 -> complex64); written into `out`
+        (contiguous, that shape) when given."""
         if obj.ndim != 3 or tuple(obj.shape[1:]) != (self.n, self.n):
             raise ValueError(f"expected [nz, {self.n}, {self.n}], got {tuple(obj.shape)}")
-        return self._run("htamd_tomo_R", obj, (self.ntheta, obj.shape[0], self.n))
+        return self._run("htamd_tomo_R", obj, (self.ntheta, obj.shape[0], self.n), out)
 
-    def RT(self, data):
-        """Adjoint Radon transform (tomo.py:88-116): [ntheta, nz', n] -> [nz', n, n]."""
+    def RT(self, data, out=None):
+        """Adjoint Radon transform (tomo.py:88-116): [ntheta, nz', n] -> [nz', n, n];
+        written into `out` (contiguous, that shape) when given."""
         if data.ndim != 3 or data.shape[0] != self.ntheta or data.shape[2] != self.n:
             raise ValueError(f"expected [{self.ntheta}, nz, {self.n}], got {tuple(data.shape)}")
-        return self._run("htamd_tomo_RT", data, (data.shape[1], self.n, self.n))
+        return self._run("htamd_tomo_RT", data, (data.shape[1], self.n, self.n), out)
 
     def _filter_sino(self, data, filter_name):
         """Ramp / Shepp-Logan / Parzen filter along the detector axis
