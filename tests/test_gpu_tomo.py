@@ -141,3 +141,27 @@ def test_fbp_and_rec_tomo_shapes(gpu):
     u = g.rec_tomo(d, 8)
     r0 = float(torch.linalg.norm(d))
     assert float(torch.linalg.norm(g.R(u) - d)) < 0.2 * r0
+
+
+def test_R_RT_into_given_output(gpu):
+    """R / RT written into a caller's contiguous tensor equal the allocating
+    calls bit for bit; a wrong shape, dtype or a non-contiguous view is
+    rejected before any launch."""
+    from holotomocupy_amd import Tomo
+    print("[case] R/RT into a given output", flush=True)
+    n, nth, nz = 32, 12, 3
+    th = np.arange(nth) * np.pi / nth
+    g = Tomo(n, nz, th, 0.9)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    u = torch.randn((nz, n, n), dtype=torch.complex64, device="cuda", generator=gen)
+    d = torch.randn((nth, nz, n), dtype=torch.complex64, device="cuda", generator=gen)
+    out = torch.empty((nth, nz, n), dtype=torch.complex64, device="cuda")
+    assert g.R(u, out=out) is out and torch.equal(out, g.R(u))
+    back = torch.empty((nz, n, n), dtype=torch.complex64, device="cuda")
+    assert g.RT(d, out=back) is back and torch.equal(back, g.RT(d))
+    with pytest.raises(ValueError):
+        g.R(u, out=torch.empty((nth, nz, n + 1), dtype=torch.complex64, device="cuda"))
+    with pytest.raises(ValueError):
+        g.R(u, out=torch.empty((nth, nz, n), dtype=torch.float32, device="cuda"))
+    with pytest.raises(ValueError):
+        g.RT(d, out=torch.empty((n, nz, n), dtype=torch.complex64, device="cuda").transpose(0, 1))
