@@ -205,7 +205,8 @@ __global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
     };
     // equal directions: full unroll keeps the three arrays in registers; the
     // general and pair forms' four or five arrays measure faster in scratch
-    // with a compact loop
+    // with a compact loop (fully unrolled at 2048: 256 VGPRs, one wave per
+    // SIMD, 23.0 -> 27.5 ms for the general form at 16 angles, 4 distances)
     if constexpr (SAME) {
 #pragma unroll
       for (int e = 0; e < E; ++e) chain(e);
