@@ -53,6 +53,8 @@ SIGNATURES = {
     "htamd_tomo_workspace_size": [c_vp, c_i64, ctypes.POINTER(c_i64)],
     "htamd_tomo_R": [c_vp, c_vp, c_int, c_vp, c_i64, c_vp, c_i64, c_vp],
     "htamd_tomo_RT": [c_vp, c_vp, c_int, c_vp, c_i64, c_vp, c_i64, c_vp],
+    "htamd_tomo_filter": [c_vp, c_vp, c_int, c_int, c_vp, c_i64, c_vp],
+    "htamd_tomo_fbp": [c_vp, c_vp, c_int, c_int, c_vp, c_i64, c_vp, c_i64, c_vp],
     "htamd_shift_plan_create": [c_i64, c_i64, c_i64, c_i64, c_int, ctypes.POINTER(c_vp)],
     "htamd_shift_plan_destroy": [c_vp],
     "htamd_bspline_prefilter": [c_vp, c_vp, c_vp, c_i64, c_vp],

@@ -50,6 +50,10 @@ struct TomoArgs {
   float2* W = nullptr;             // [nz, n, 2n]
   float2* G = nullptr;             // [nz, 2n, 2n]
   float2* Sh = nullptr;            // [ntheta n, nz] transformed sinogram (RT)
+  // per-bin factor of the adjoint's line transform (NULL = none): the
+  // detector filter of filtered back-projection, shifted by n/2 to the bins
+  // of the c1d-modulated transform and scaled by sqrt(n / ntheta)
+  const float* filt = nullptr;
 };
 
 __device__ __forceinline__ float sgnf(int k) { return (k & 1) ? -1.0f : 1.0f; }
@@ -286,7 +290,46 @@ __global__ __launch_bounds__(256) void tomo_linefft_k(TomoArgs a, const TI* __re
   if (valid) {
     for (int e = 0; e < E; ++e) {
       const int tt = t + e * T;
-      a.Sh[((int64_t)th * N + tt) * a.nz + z] = cscale(s[fpad(tt)], -sgnf(tt));
+      const float sc = a.filt ? -sgnf(tt) * a.filt[tt] : -sgnf(tt);
+      a.Sh[((int64_t)th * N + tt) * a.nz + z] = cscale(s[fpad(tt)], sc);
+    }
+  }
+}
+
+// ---- detector filter alone (Tomo._filter_sino): IFFT(h FFT(line)) -----------
+// h already holds the 1/N of the inverse transform.  in may equal out.
+template <int N, int E, typename TIO>
+__global__ __launch_bounds__(256) void tomo_filter_k(const TIO* in, TIO* out, int64_t lines,
+                                                     const float2* __restrict__ tw, const float* __restrict__ h) {
+  constexpr int T = N / E, ROWS = 256 / T;
+  __shared__ float2 lds[ROWS * fft_buf<N>()];
+  const int r = threadIdx.x / T, t = threadIdx.x % T;
+  const int64_t line = (int64_t)blockIdx.x * ROWS + r;
+  const bool valid = line < lines;
+  float2* s = lds + r * fft_buf<N>();
+  FftTw<N, E> twr;
+  fft_load_twiddles<N, E>(twr, t, tw);
+  const TIO* src = in + (valid ? line : 0) * N;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = t + e * T;
+    s[fpad(k)] = valid ? ld_c<TIO>(src, k) : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lds<N, E, false>(s, t, twr);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = t + e * T;
+    s[fpad(k)] = cscale(s[fpad(k)], h[k]);
+  }
+  __syncthreads();
+  fft_lds<N, E, true>(s, t, twr);
+  if (valid) {
+    TIO* dst = out + line * N;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = t + e * T;
+      st_c<TIO>(dst, k, s[fpad(k)]);
     }
   }
 }
@@ -464,6 +507,30 @@ static hipError_t tomo_adj_n(const TomoArgs& a, const TIO* data, TIO* out, hipSt
   return hipGetLastError();
 }
 
+template <int N, typename TIO>
+static hipError_t tomo_filter_n(const TIO* in, TIO* out, int64_t lines, const float2* tw, const float* h,
+                                hipStream_t st) {
+  constexpr int E = N >= 8 ? 8 : N;
+  constexpr int RL = 256 / (N / E);
+  if (lines == 0) return hipSuccess;
+  hipLaunchKernelGGL((tomo_filter_k<N, E, TIO>), dim3((unsigned)((lines + RL - 1) / RL)), dim3(256), 0, st, in, out,
+                     lines, tw, h);
+  return hipGetLastError();
+}
+
+template <typename TIO>
+static hipError_t tomo_filter_dispatch(int n, const void* in, void* out, int64_t lines, const float2* tw,
+                                       const float* h, hipStream_t st) {
+#define HTAMD_TFILT_CASE(NN) \
+  case NN: return tomo_filter_n<NN, TIO>((const TIO*)in, (TIO*)out, lines, tw, h, st);
+  switch (n) {
+    HTAMD_TFILT_CASE(8) HTAMD_TFILT_CASE(16) HTAMD_TFILT_CASE(32) HTAMD_TFILT_CASE(64) HTAMD_TFILT_CASE(128)
+    HTAMD_TFILT_CASE(256) HTAMD_TFILT_CASE(512) HTAMD_TFILT_CASE(1024)
+    default: return hipErrorInvalidValue;
+  }
+#undef HTAMD_TFILT_CASE
+}
+
 template <typename TIO>
 static hipError_t tomo_dispatch(bool adjoint, const TomoArgs& a, const void* in, void* out, hipStream_t st) {
 #define HTAMD_TOMO_CASE(NN)                                                               \
@@ -498,7 +565,30 @@ struct htamd_tomo_plan {
   int2* so = nullptr;
   float2* sf = nullptr;
   int nitems = 0, ts = 0;
+  // detector filters (ramp, Shepp-Logan, Parzen): [3][n] with the 1/n of the
+  // inverse transform for htamd_tomo_filter, and [3][n] shifted by n/2 and
+  // scaled by sqrt(n / ntheta) for htamd_tomo_fbp
+  float* filt = nullptr;
+  float* filt_fbp = nullptr;
 };
+
+namespace htamd {
+
+constexpr int kTomoFilters = 3;
+
+// h[k] on fftfreq(n) (src/holotomocupy/tomo.py:131-155), FP64:
+// ramp 4n|f|, Shepp-Logan 4n|f| sinc(f), Parzen 4n|f| w(u), u = 2 (4n|f|),
+// w = 1 - 6u^2 + 6u^3 (u <= 1/2) else 2 (1 - u)^3 -- the reference's u.
+static double tomo_filter_h(int kind, int64_t k, int64_t n) {
+  const double f = (double)(k < (n + 1) / 2 ? k : k - n) / (double)n;
+  const double af = std::fabs(f) * 4.0 * (double)n;
+  if (kind == 0) return af;
+  if (kind == 1) return f == 0.0 ? af : af * std::sin(M_PI * f) / (M_PI * f);
+  const double u = 2.0 * af;
+  return af * (u <= 0.5 ? 1.0 - 6.0 * u * u + 6.0 * u * u * u : 2.0 * (1.0 - u) * (1.0 - u) * (1.0 - u));
+}
+
+}  // namespace htamd
 
 extern "C" {
 
@@ -615,6 +705,14 @@ int htamd_tomo_plan_create(int64_t n, int64_t ntheta, const double* theta, const
       phi[(size_t)(i * n + j)] = (float)v;
     }
   }
+  std::vector<float> filt((size_t)(kTomoFilters * n)), filt_fbp((size_t)(kTomoFilters * n));
+  const double norm_const = std::sqrt((double)n / (double)ntheta);
+  for (int kind = 0; kind < kTomoFilters; ++kind)
+    for (int64_t k = 0; k < n; ++k) {
+      filt[(size_t)(kind * n + k)] = (float)(tomo_filter_h(kind, k, n) / (double)n);
+      // FFT(c1d d)[k] picks up h at bin (k - n/2) mod n
+      filt_fbp[(size_t)(kind * n + k)] = (float)(tomo_filter_h(kind, (k + n / 2) % n, n) * norm_const);
+    }
   auto twid = [](int64_t N) {
     std::vector<float2> t((size_t)N);
     for (int64_t k = 0; k < N; ++k) {
@@ -632,7 +730,8 @@ int htamd_tomo_plan_create(int64_t n, int64_t ntheta, const double* theta, const
       (e = up(&p->tw1, twid(n))) != hipSuccess || (e = up(&p->tb, tb)) != hipSuccess ||
       (e = up(&p->tw, tw)) != hipSuccess || (e = up(&p->rp, rp)) != hipSuccess || (e = up(&p->csw, csw)) != hipSuccess ||
       (e = up(&p->items, items)) != hipSuccess ||
-      (e = up(&p->so, so)) != hipSuccess || (e = up(&p->sf, sf)) != hipSuccess) {
+      (e = up(&p->so, so)) != hipSuccess || (e = up(&p->sf, sf)) != hipSuccess ||
+      (e = up(&p->filt, filt)) != hipSuccess || (e = up(&p->filt_fbp, filt_fbp)) != hipSuccess) {
     htamd_tomo_plan_destroy(p);
     return hip_fail(e, "tomo_plan_create");
   }
@@ -652,6 +751,8 @@ int htamd_tomo_plan_destroy(htamd_tomo_plan* p) {
   (void)hipFree(p->items);
   (void)hipFree(p->so);
   (void)hipFree(p->sf);
+  (void)hipFree(p->filt);
+  (void)hipFree(p->filt_fbp);
   delete p;
   return HTAMD_OK;
 }
@@ -663,7 +764,7 @@ int htamd_tomo_workspace_size(const htamd_tomo_plan* p, int64_t nz, int64_t* byt
 }
 
 static int tomo_run(const htamd_tomo_plan* p, bool adjoint, const void* in, int is_complex, void* out,
-                    int64_t nz, void* ws, int64_t ws_bytes, void* stream, const char* what) {
+                    int64_t nz, void* ws, int64_t ws_bytes, void* stream, const char* what, int filter = -1) {
   if (!p) return fail(HTAMD_ERR_ARG, std::string(what) + ": plan is NULL");
   if (nz < 0) return fail(HTAMD_ERR_ARG, std::string(what) + ": negative slice count");
   if (nz == 0) return HTAMD_OK;
@@ -697,6 +798,7 @@ static int tomo_run(const htamd_tomo_plan* p, bool adjoint, const void* in, int 
   a.W = (float2*)ws;
   a.G = (float2*)ws + nz * p->n * 2 * p->n;
   a.Sh = a.G + nz * 4 * p->n * p->n;
+  if (filter >= 0) a.filt = p->filt_fbp + (int64_t)filter * p->n;
   hipError_t e = is_complex ? tomo_dispatch<float2>(adjoint, a, in, out, (hipStream_t)stream)
                             : tomo_dispatch<float>(adjoint, a, in, out, (hipStream_t)stream);
   return e == hipSuccess ? HTAMD_OK : hip_fail(e, what);
@@ -710,6 +812,31 @@ int htamd_tomo_R(const htamd_tomo_plan* p, const void* obj, int is_complex, void
 int htamd_tomo_RT(const htamd_tomo_plan* p, const void* data, int is_complex, void* out, int64_t nz,
                   void* workspace, int64_t workspace_bytes, void* stream) {
   return tomo_run(p, true, data, is_complex, out, nz, workspace, workspace_bytes, stream, "tomo_RT");
+}
+
+int htamd_tomo_fbp(const htamd_tomo_plan* p, const void* data, int is_complex, int filter, void* out, int64_t nz,
+                   void* workspace, int64_t workspace_bytes, void* stream) {
+  if (filter < 0 || filter >= kTomoFilters)
+    return fail(HTAMD_ERR_ARG, "tomo_fbp: filter must be 0 (ramp), 1 (shepp) or 2 (parzen)");
+  return tomo_run(p, true, data, is_complex, out, nz, workspace, workspace_bytes, stream, "tomo_fbp", filter);
+}
+
+int htamd_tomo_filter(const htamd_tomo_plan* p, const void* data, int is_complex, int filter, void* out, int64_t nz,
+                      void* stream) {
+  if (!p) return fail(HTAMD_ERR_ARG, "tomo_filter: plan is NULL");
+  if (filter < 0 || filter >= kTomoFilters)
+    return fail(HTAMD_ERR_ARG, "tomo_filter: filter must be 0 (ramp), 1 (shepp) or 2 (parzen)");
+  if (nz < 0) return fail(HTAMD_ERR_ARG, "tomo_filter: negative slice count");
+  if (nz == 0) return HTAMD_OK;
+  if (!data || !out) return fail(HTAMD_ERR_ARG, "tomo_filter: NULL buffer");
+  int dev = -1;
+  (void)hipGetDevice(&dev);
+  if (dev != p->device) return fail(HTAMD_ERR_ARG, "tomo_filter: plan belongs to another device");
+  const float* h = p->filt + (int64_t)filter * p->n;
+  const int64_t lines = p->ntheta * nz;
+  hipError_t e = is_complex ? tomo_filter_dispatch<float2>((int)p->n, data, out, lines, p->tw1, h, (hipStream_t)stream)
+                            : tomo_filter_dispatch<float>((int)p->n, data, out, lines, p->tw1, h, (hipStream_t)stream);
+  return e == hipSuccess ? HTAMD_OK : hip_fail(e, "tomo_filter");
 }
 
 }  // extern "C"

@@ -418,10 +418,10 @@ class Rec(BHLoop):
 
     def _tomo_slab(self):
         """Slices per R / RT call on device-resident arrays: as many as keep the
-        oversampled grid workspace (32 nobj^2 bytes per slice) within 4 GiB, a
-        multiple of 32 (the RT gather's slice block) when that many fit, and at
-        least nchunk."""
-        per_slice = 32 * self.nobj * self.nobj
+        Radon workspace (htamd_tomo_workspace_size: 6 nobj^2 + ntheta nobj
+        complex64 per slice) within 4 GiB, a multiple of 32 (the RT gather's
+        slice block) when that many fit, and at least nchunk."""
+        per_slice = 8 * (6 * self.nobj * self.nobj + self.ntheta * self.nobj)
         slab = max(1, (4 << 30) // per_slice)
         if slab >= 32:
             slab -= slab % 32
@@ -443,7 +443,8 @@ class Rec(BHLoop):
         tomo = self._tomo()
         if isinstance(obj, torch.Tensor) and obj.is_cuda and isinstance(out, torch.Tensor) and out.is_cuda:
             slab = self._tomo_slab()
-            if slab >= obj.shape[0] and out.is_contiguous() and out.dtype == obj.dtype:
+            if (slab >= obj.shape[0] and out.is_contiguous() and out.dtype == obj.dtype
+                    and obj.dtype in (torch.float32, torch.complex64)):
                 return tomo.R(obj, out=out)  # one slab: straight into out
             for z0 in range(0, obj.shape[0], slab):
                 z1 = min(obj.shape[0], z0 + slab)
@@ -469,7 +470,8 @@ class Rec(BHLoop):
                 z1 = min(gradu.shape[0], z0 + slab)
                 src = gradproj if (z0, z1) == (0, gradu.shape[0]) else gradproj[:, z0:z1].contiguous()
                 dst = gradu[z0:z1]  # whole slices: contiguous when gradu is
-                if dst.is_contiguous() and dst.dtype == src.dtype:
+                if dst.is_contiguous() and dst.dtype == src.dtype and src.dtype in (torch.float32,
+                                                                                     torch.complex64):
                     tomo.RT(src, out=dst)
                 else:
                     dst[:] = tomo.RT(src)

@@ -306,9 +306,15 @@ class GpuExecutor:
         """Device bytes a request holds while in flight beyond the arenas: the
         packed batch and output of a propagation, the result copies of a
         gradient, the scalar of a bilinear form."""
-        if isinstance(req, TomoRequest):  # stacked input, output, oversampled grid
-            nth = len(req.theta)
-            return req.nangles * (req.n * req.n * 8 + nth * req.n * 8 + 32 * req.n * req.n)
+        if isinstance(req, TomoRequest):
+            # per slice: the stacked copy of the input, the output and the
+            # Radon workspace (htamd_tomo_workspace_size: 6 n^2 + ntheta n
+            # complex64), with the request's own input counted once more
+            nth, n = len(req.theta), req.n
+            esz = 8 if req.x.is_complex() else 4
+            slab_in = n * n * esz if req.op == "R" else nth * n * esz
+            slab_out = nth * n * esz if req.op == "R" else n * n * esz
+            return req.nangles * (2 * slab_in + slab_out + 8 * (6 * n * n + nth * n))
         g = req.geometry
         if isinstance(req, OperatorRequest):
             return 2 * req.nangles * g.nz * g.n * 8
@@ -386,6 +392,7 @@ class GpuExecutor:
         geo, _, op, j = key
         b = self._bucket(geo)
         with torch.cuda.stream(s):
+            self._wait_inputs(batch, s)
             for r in batch:
                 r.x.record_stream(s)
 
@@ -414,6 +421,7 @@ class GpuExecutor:
         op = key[4]
         tomo = self._tomo(key)
         with torch.cuda.stream(s):
+            self._wait_inputs(batch, s)
             for r in batch:
                 r.x.record_stream(s)
             zdim = 0 if op == "R" else 1
@@ -437,6 +445,7 @@ class GpuExecutor:
         b = self._bucket(geo)
         outs = []
         with torch.cuda.stream(s):
+            self._wait_inputs(batch, s)
             mag = b["rec"].eff_demagnifications[0]
             for r in batch:
                 for d in (r.x, r.y) + ((r.z,) if r.z is not None else ()):
@@ -468,6 +477,7 @@ class GpuExecutor:
         P = len(batch)
         slot = self.slots[si]
         with torch.cuda.stream(s):
+            self._wait_inputs(batch, s)
             st = self._gradient_arena(si, b, geo, A, P)
             mag = b["rec"].eff_demagnifications[0]
             o = 0
@@ -522,9 +532,24 @@ class GpuExecutor:
         return ev, outs
 
     def prepare(self, req):
-        """Called in the submitting thread: its current stream is where the
-        request's results will be used."""
-        req.consumer = self.torch.cuda.current_stream(self.device)
+        """Called in the submitting thread.  Its current stream is where the
+        request's inputs were produced and where its results will be used: an
+        event recorded there now orders every executor-stream read of the
+        inputs after the work the submitter had queued (the executor streams
+        do not otherwise synchronise with the submitter's stream)."""
+        torch = self.torch
+        req.consumer = torch.cuda.current_stream(self.device)
+        ev = torch.cuda.Event()
+        ev.record(req.consumer)
+        req.ready = ev
+
+    @staticmethod
+    def _wait_inputs(batch, s):
+        """Make stream s wait for each request's inputs (see prepare)."""
+        for r in batch:
+            ev = getattr(r, "ready", None)
+            if ev is not None:
+                s.wait_event(ev)
 
     def done(self, handle):
         return handle[0].query()

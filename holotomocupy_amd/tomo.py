@@ -104,31 +104,48 @@ class Tomo:
             raise ValueError(f"expected [{self.ntheta}, nz, {self.n}], got {tuple(data.shape)}")
         return self._run("htamd_tomo_RT", data, (data.shape[1], self.n, self.n), out)
 
+    _FILTERS = {"ramp": 0, "shepp": 1, "parzen": 2}
+
+    def _filter_kind(self, filter_name):
+        if filter_name not in self._FILTERS:
+            raise ValueError(f"Unknown filter '{filter_name}'. Choose: ramp, shepp, parzen.")
+        return self._FILTERS[filter_name]
+
+    @staticmethod
+    def _sino_dtype(data):
+        """complex64 for a complex sinogram, float32 otherwise (host or device)."""
+        if isinstance(data, torch.Tensor):
+            return torch.complex64 if data.is_complex() else torch.float32
+        return torch.complex64 if np.iscomplexobj(data) else torch.float32
+
     def _filter_sino(self, data, filter_name):
         """Ramp / Shepp-Logan / Parzen filter along the detector axis
-        (tomo.py:118-162); data preparation, run with the device FFT."""
-        n = self.n
-        f = torch.fft.fftfreq(n, device=self.device).float()
-        af = f.abs() * 4 * n
-        if filter_name == "ramp":
-            h = af
-        elif filter_name == "shepp":
-            h = af * torch.sinc(f)
-        elif filter_name == "parzen":
-            u = 2 * af
-            h = af * torch.where(u <= 0.5, 1 - 6 * u ** 2 + 6 * u ** 3, 2 * (1 - u) ** 3)
-        else:
-            raise ValueError(f"Unknown filter '{filter_name}'. Choose: ramp, shepp, parzen.")
-        res = torch.fft.ifft(torch.fft.fft(data, dim=-1) * h, dim=-1)
-        return res.to(data.dtype) if data.is_complex() else res.real.to(data.dtype)
+        (tomo.py:118-162): IFFT(h FFT(data)) in one LDS pass per line
+        (htamd_tomo_filter); same dtype as data."""
+        kind = self._filter_kind(filter_name)
+        x = _lib.as_device(data, self._sino_dtype(data), self.device)
+        if x.ndim != 3 or x.shape[0] != self.ntheta or x.shape[2] != self.n:
+            raise ValueError(f"expected [{self.ntheta}, nz, {self.n}], got {tuple(x.shape)}")
+        out = torch.empty_like(x)
+        _lib.call("htamd_tomo_filter", self._plan, _lib.ptr(x), 1 if x.is_complex() else 0, kind, _lib.ptr(out),
+                  x.shape[1], _lib.stream_ptr())
+        return out
 
     def fbp(self, data, filter_name="ramp"):
-        """Filtered back-projection (tomo.py:164-186)."""
-        norm_const = np.float32(np.sqrt(self.n / self.ntheta))
-        data = _lib.as_device(data, data.dtype if isinstance(data, torch.Tensor) else torch.float32, self.device)
-        res = self.RT(self._filter_sino(data, filter_name))
-        res *= float(norm_const)
-        return res
+        """Filtered back-projection (tomo.py:164-186): RT(filter(data)) sqrt(n/ntheta),
+        with the filter applied inside RT's own line transform (htamd_tomo_fbp)."""
+        kind = self._filter_kind(filter_name)
+        x = _lib.as_device(data, self._sino_dtype(data), self.device)
+        if x.ndim != 3 or x.shape[0] != self.ntheta or x.shape[2] != self.n:
+            raise ValueError(f"expected [{self.ntheta}, nz, {self.n}], got {tuple(x.shape)}")
+        nz = x.shape[1]
+        if nz > self._nz:
+            raise ValueError(f"{nz} slices exceed the {self._nz} this Tomo was built for")
+        out = torch.empty((nz, self.n, self.n), dtype=x.dtype, device=self.device)
+        ws = self._workspace(nz)
+        _lib.call("htamd_tomo_fbp", self._plan, _lib.ptr(x), 1 if x.is_complex() else 0, kind, _lib.ptr(out), nz,
+                  _lib.ptr(ws), ws.numel(), _lib.stream_ptr())
+        return out
 
     def rec_tomo(self, d, niter=1):
         """Conjugate-gradient tomography for an initial guess (tomo.py:188-210)."""

@@ -319,6 +319,20 @@ int htamd_tomo_R(const htamd_tomo_plan* plan, const void* obj, int is_complex, v
 int htamd_tomo_RT(const htamd_tomo_plan* plan, const void* data, int is_complex, void* out, int64_t nz,
                   void* workspace, int64_t workspace_bytes, void* stream);
 
+/* Detector-axis filter of filtered back-projection: out = IFFT(h FFT(data)) along the last axis of
+ * data [ntheta, nz, n] (float32: real part kept, as the reference does for a float32 sinogram), with
+ * h on fftfreq(n): filter 0 = ramp 4n|f|, 1 = Shepp-Logan 4n|f| sinc(f), 2 = Parzen 4n|f| w(u) with the
+ * reference's u = 2 (4n|f|).  h is built in FP64 per plan.  out may equal data; no workspace.
+ * Replaces Tomo._filter_sino (src/holotomocupy/tomo.py:118-162). */
+int htamd_tomo_filter(const htamd_tomo_plan* plan, const void* data, int is_complex, int filter, void* out,
+                      int64_t nz, void* stream);
+/* Filtered back-projection out [nz, n, n] = RT(filter(data)) sqrt(n / ntheta) in the passes of RT alone:
+ * the filter multiplies the bins of RT's own line transform (shifted by n/2 for its c1d modulation), so
+ * the sinogram is transformed once.  Workspace as htamd_tomo_RT.
+ * Replaces Tomo.fbp (src/holotomocupy/tomo.py:164-186). */
+int htamd_tomo_fbp(const htamd_tomo_plan* plan, const void* data, int is_complex, int filter, void* out,
+                   int64_t nz, void* workspace, int64_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,193 @@
+"""Served results of a mixed-size request burst against the FP64 oracle.
+
+The burst holds what the serving mix draws from: BH gradient requests at
+2048^2 (4 distances, object plane 1024^2), 1024^2 (3 distances, 512^2) and
+256^2 (2 distances, 256^2), a bilinear-form request at 512^2, D / DT requests
+at 1024^2 and R / RT requests at 256^2.  All are submitted at once, so the
+gradient requests of one geometry share a fused launch and every kind runs
+beside the others.  Each served result is compared with the oracle on the
+same inputs (gradients: probe, object and positions; for the 16-angle 2048^2
+request, whose first angle repeats the 1-angle request, the object and
+position gradients of that angle), at the bars of the operator tests: 1e-5
+relative L2 (probe, object, D / DT, R / RT), 1e-4 for positions and the
+bilinear form.  A second test submits requests whose inputs are still being
+written by queued kernels on the submitting stream."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+Z1 = (5.110e-3, 5.464e-3, 6.879e-3, 9.817e-3)
+
+
+def rel(a, b):
+    a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    return float(np.linalg.norm(a.astype(np.complex128) - b) / np.linalg.norm(b))
+
+
+def report(name, err, bar):
+    print(f"[serve-oracle] {name}: {err:.2e} (bar {bar:.0e})", flush=True)
+    assert err < bar, name
+
+
+def geometry(n, nobj, nd):
+    from holotomocupy_amd.serve import Geometry
+    return Geometry(n, n, nobj, nobj, nd, z1=Z1, detector_pixelsize=1.4760147601476e-6 * 2 * (2048 // n))
+
+
+def oracle_cascade(geo, B, data_size, rho):
+    from holotomocupy_amd.rec import physics
+    from oracle import cascade as oc
+    from oracle import propagation as op
+    from oracle import shift as osh
+    op.WORKERS = osh.WORKERS = 16
+    nd = geo.ndist
+    ph = physics(geo.energy, np.asarray(geo.z1[:nd]), geo.focustodetectordistance, geo.detector_pixelsize)
+    return oc.Cascade(op.Propagation(geo.n, geo.nz, B, nd, ph.wavelength, ph.voxelsize, ph.distance),
+                      osh.Shift(geo.n, geo.nobj, geo.nz, geo.nzobj), nd, geo.nz, geo.n, geo.nzobj, geo.nobj,
+                      data_size, np.tile(1 / ph.norm_magnifications, (B, 1)), rho=rho), ph
+
+
+def host_inputs(rng, geo, A):
+    from holotomocupy_amd import phantom
+    prb = phantom.probe(rng, geo.ndist, geo.nz, geo.n)
+    proj = phantom.projections(rng, A, geo.nzobj, geo.nobj)
+    pos = phantom.positions(rng, A, geo.ndist, 3.0)
+    return prb, proj, pos
+
+
+def amplitudes(cas, prb, proj, pos, rng):
+    amp = np.abs(cas.forward(prb.astype(np.complex128), proj.astype(np.complex128), pos.astype(np.float64)))
+    return np.sqrt(rng.poisson(1e4 * amp ** 2) / 1e4).astype(np.float32)
+
+
+def wide(*v):
+    return [a.astype(np.complex128) if np.iscomplexobj(a) else a.astype(np.float64) for a in v]
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def test_served_mix_against_oracle(gpu):
+    from holotomocupy_amd.serve import BHServer, BilinearRequest, GradientRequest, OperatorRequest, TomoRequest
+    from oracle import propagation as op
+    from oracle import tomo as ot
+    print("[case] served mix 256..2048 against the FP64 oracle", flush=True)
+    rng = np.random.default_rng(1005)
+    rho = (1.0, 0.05, 0.02)
+    G2048, G1024, G256, G512 = geometry(2048, 1024, 4), geometry(1024, 512, 3), geometry(256, 256, 2), \
+        geometry(512, 512, 2)
+
+    # 2048^2: a 1-angle request and a 16-angle request whose angle 0 repeats it
+    p1, j1, s1 = host_inputs(rng, G2048, 1)
+    cas1, _ = oracle_cascade(G2048, 1, 1800 * 4 * 2048 * 2048, rho)
+    d1 = amplitudes(cas1, p1, j1, s1, rng)
+    x1 = [(p1 * 0.9).astype(np.complex64), (j1 * 0.8).astype(np.complex64),
+          (s1 + rng.uniform(-0.3, 0.3, s1.shape)).astype(np.float32)]
+    _, j16, s16 = host_inputs(rng, G2048, 16)
+    j16[0], s16[0] = x1[1][0], x1[2][0]
+    d16 = np.concatenate([d1, rng.uniform(0.5, 1.5, (15, 4, 2048, 2048)).astype(np.float32)])
+    # 1024^2 (3 distances, 2 angles) and 256^2 (2 distances, 8 angles)
+    p2, j2, s2 = host_inputs(rng, G1024, 2)
+    cas2, _ = oracle_cascade(G1024, 2, 720 * 3 * 1024 * 1024, rho)
+    d2 = amplitudes(cas2, p2, j2, s2, rng)
+    x2 = [(p2 * 0.9).astype(np.complex64), (j2 * 0.8).astype(np.complex64), s2]
+    p3, j3, s3 = host_inputs(rng, G256, 8)
+    cas3, _ = oracle_cascade(G256, 8, 8 * 2 * 256 * 256, rho)
+    d3 = amplitudes(cas3, p3, j3, s3, rng)
+    x3 = [(p3 * 1.1).astype(np.complex64), (j3 * 0.7).astype(np.complex64), s3]
+    # bilinear form at 512^2 (2 distances, 2 angles)
+    p4, j4, s4 = host_inputs(rng, G512, 2)
+    cas4, _ = oracle_cascade(G512, 2, 2 * 2 * 512 * 512, (1.0, 1.0, 1.0))
+    d4 = amplitudes(cas4, p4, j4, s4, rng)
+    y4 = [(0.05 * p4).astype(np.complex64), (0.05 * np.roll(j4, 5, axis=2)).astype(np.complex64),
+          rng.uniform(-0.3, 0.3, s4.shape).astype(np.float32)]
+    # operators: D_1 / DT_2 of 1024^2 fields, R / RT at 256^2 with 64 angles
+    fields = (rng.standard_normal((3, 1024, 1024)) + 1j * rng.standard_normal((3, 1024, 1024))).astype(np.complex64)
+    theta = tuple(float(t) for t in np.arange(64) * np.pi / 64)
+    slab = (rng.standard_normal((4, 256, 256)) + 1j * rng.standard_normal((4, 256, 256))).astype(np.complex64)
+    sino = (rng.standard_normal((64, 4, 256)) + 1j * rng.standard_normal((64, 4, 256))).astype(np.complex64)
+
+    def greq(geo, x, d, ds):
+        return GradientRequest(geo, dev(x[0]), dev(x[1]), dev(x[2]), dev(d), data_size=ds, rho=rho)
+
+    reqs = [greq(G2048, x1, d1, 1800 * 4 * 2048 * 2048),
+            GradientRequest(G2048, dev(x1[0]), dev(j16), dev(s16), dev(d16), data_size=1800 * 4 * 2048 * 2048,
+                            rho=rho),
+            greq(G1024, x2, d2, 720 * 3 * 1024 * 1024),
+            greq(G256, x3, d3, 8 * 2 * 256 * 256),
+            BilinearRequest(G512, {"prb": dev(p4), "proj": dev(j4), "pos": dev(s4)},
+                            {"prb": dev(y4[0]), "proj": dev(y4[1]), "pos": dev(y4[2])}, None, dev(d4),
+                            2 * 2 * 512 * 512),
+            OperatorRequest(G1024, "D", 1, dev(fields)),
+            OperatorRequest(G1024, "DT", 2, dev(fields[:2])),
+            TomoRequest(256, theta, 0.0, "R", dev(slab)),
+            TomoRequest(256, theta, 0.0, "RT", dev(sino))]
+    torch.cuda.synchronize()
+    srv = BHServer(hbm_budget=64 << 30)
+    futs = srv.submit_many(reqs)
+    res = [f.result(timeout=300) for f in futs]
+    srv.close()
+    print("[serve-oracle] served; checking against the oracle", flush=True)
+
+    g1 = cas1.gradients(*wide(*x1), d1.astype(np.float64))
+    for name, got, ref, bar in zip(("probe", "object", "positions"), res[0], g1, (1e-5, 1e-5, 1e-4)):
+        report(f"2048^2 1 angle {name}", rel(got, ref), bar)
+    report("2048^2 16 angles: object of angle 0", rel(res[1][1][0], g1[1][0]), 1e-5)
+    report("2048^2 16 angles: positions of angle 0", rel(res[1][2][0], g1[2][0]), 1e-4)
+    g2 = cas2.gradients(*wide(*x2), d2.astype(np.float64))
+    for name, got, ref, bar in zip(("probe", "object", "positions"), res[2], g2, (1e-5, 1e-5, 1e-4)):
+        report(f"1024^2 {name}", rel(got, ref), bar)
+    g3 = cas3.gradients(*wide(*x3), d3.astype(np.float64))
+    for name, got, ref, bar in zip(("probe", "object", "positions"), res[3], g3, (1e-5, 1e-5, 1e-4)):
+        report(f"256^2 {name}", rel(got, ref), bar)
+    yw = wide(*y4)
+    h = cas4.hessian(wide(p4, j4, s4), yw, yw, d4.astype(np.float64))
+    report("512^2 bilinear form H(y, y)", abs(float(res[4]) - h) / abs(h), 1e-4)
+    _, ph2 = oracle_cascade(G1024, 1, 1, rho)
+    prop = op.Propagation(1024, 1024, 3, 3, ph2.wavelength, ph2.voxelsize, ph2.distance)
+    report("1024^2 D_1", rel(res[5], prop.D(fields.astype(np.complex128), 1)), 1e-5)
+    report("1024^2 DT_2", rel(res[6], prop.DT(fields[:2].astype(np.complex128), 2)), 1e-5)
+    tomo = ot.Tomo(256, 4, np.asarray(theta), 0.0)
+    report("256^2 R", rel(res[7], tomo.R(slab.astype(np.complex128))), 1e-5)
+    report("256^2 RT", rel(res[8], tomo.RT(sino.astype(np.complex128))), 1e-5)
+
+
+def test_inputs_still_being_written_are_waited_for(gpu):
+    """Inputs are produced by a long chain of kernels queued on the
+    submitting stream and submitted without a synchronisation: the served
+    results equal those of the same request submitted after a sync."""
+    from holotomocupy_amd.serve import BHServer, GradientRequest, OperatorRequest
+    print("[case] serve: inputs written by queued kernels", flush=True)
+    geo = geometry(256, 256, 2)
+    rng = np.random.default_rng(55)
+    prb, proj, pos = host_inputs(rng, geo, 4)
+    data = rng.uniform(0.5, 1.5, (4, 2, 256, 256)).astype(np.float32)
+    srv = BHServer(hbm_budget=8 << 30)
+    ref = srv.submit(GradientRequest(geo, dev(prb), dev(proj), dev(pos), dev(data), data_size=1e6)).result(60)
+    ref = [t.clone() for t in ref]
+    fields = dev((rng.standard_normal((2, 256, 256)) + 1j * rng.standard_normal((2, 256, 256))).astype(np.complex64))
+    ref_d = srv.submit(OperatorRequest(geo, "D", 0, fields)).result(60).clone()
+    torch.cuda.synchronize()
+    # destinations start as zeros; a slow kernel chain runs first, then the
+    # true values are copied in -- all queued, nothing synchronised
+    src = [dev(prb), dev(proj), dev(pos), dev(data)]
+    dst = [torch.zeros_like(t) for t in src]
+    fdst = torch.zeros_like(fields)
+    torch.cuda.synchronize()
+    m = torch.randn(4096, 4096, device="cuda") / 64.0
+    for _ in range(30):
+        m = torch.tanh(m @ m)
+    for d, s in zip(dst, src):
+        d.copy_(s + 0 * m[0, 0])
+    fdst.copy_(fields + 0 * m[0, 0])
+    f1 = srv.submit(GradientRequest(geo, *dst, data_size=1e6))
+    f2 = srv.submit(OperatorRequest(geo, "D", 0, fdst))
+    got, got_d = f1.result(60), f2.result(60)
+    torch.cuda.synchronize()
+    srv.close()
+    for a, b in zip(ref, got):
+        assert torch.equal(a, b)
+    assert torch.equal(ref_d, got_d)

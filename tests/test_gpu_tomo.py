@@ -165,3 +165,59 @@ def test_R_RT_into_given_output(gpu):
         g.R(u, out=torch.empty((nth, nz, n), dtype=torch.float32, device="cuda"))
     with pytest.raises(ValueError):
         g.RT(d, out=torch.empty((n, nz, n), dtype=torch.complex64, device="cuda").transpose(0, 1))
+
+
+@pytest.mark.parametrize("name", ["ramp", "shepp", "parzen"])
+@pytest.mark.parametrize("n,ntheta,nz,cplx", [(64, 96, 2, True), (128, 64, 3, False), (256, 32, 1, True)])
+def test_filter_and_fbp_match_oracle(gpu, name, n, ntheta, nz, cplx):
+    """_filter_sino (one LDS pass per line) and fbp (the filter folded into
+    RT's line transform) against the FP64 oracle restatement of
+    tomo.py:118-186, relative L2 <= 1e-5; a complex NumPy sinogram keeps its
+    imaginary part."""
+    from holotomocupy_amd import Tomo
+    from oracle import tomo as ot
+    print(f"[case] filter/fbp {name} n={n} ntheta={ntheta} nz={nz} complex={cplx}", flush=True)
+    rng = np.random.default_rng(n + ntheta)
+    th = angles(ntheta)
+    d = rng.standard_normal((ntheta, nz, n))
+    if cplx:
+        d = d + 1j * rng.standard_normal((ntheta, nz, n))
+    d32 = d.astype(np.complex64 if cplx else np.float32)
+    g, o = Tomo(n, nz, th, 0.9), ot.Tomo(n, nz, th, 0.9)
+    f = g._filter_sino(dev(d32), name)
+    fr = o.filter_sino(d32.astype(np.complex128 if cplx else np.float64), name)
+    assert f.dtype == (torch.complex64 if cplx else torch.float32)
+    assert rel(f.cpu().numpy(), fr) < 1e-5, name
+    rec = g.fbp(d32, name)  # host array in: dtype kept
+    assert rec.dtype == (torch.complex64 if cplx else torch.float32) and tuple(rec.shape) == (nz, n, n)
+    ref = o.fbp(d32.astype(np.complex128 if cplx else np.float64), name)
+    err = rel(rec.cpu().numpy(), ref)
+    print(f"[fbp] {name} n={n}: {err:.2e}", flush=True)
+    assert err < 1e-5, name
+    # the fused pass equals the filter followed by RT
+    assert rel(rec.cpu().numpy(), (g.RT(f) * float(np.sqrt(n / ntheta))).cpu().numpy()) < 1e-5
+    with pytest.raises(ValueError):
+        g.fbp(d32, "hann")
+
+
+def test_rec_tomo_matches_oracle(gpu):
+    """Eight CG iterations of rec_tomo (tomo.py:188-210) against the FP64
+    oracle's: relative L2 <= 1e-4 (each iteration adds the FP32 error of one
+    R and one RT, ~5e-6 each, and alpha / beta are ratios of such sums)."""
+    from holotomocupy_amd import Tomo
+    from oracle import tomo as ot
+    print("[case] rec_tomo 8 iterations n=64", flush=True)
+    n, ntheta, nz = 64, 96, 2
+    th = angles(ntheta)
+    rng = np.random.default_rng(9)
+    g, o = Tomo(n, nz, th, 0.9), ot.Tomo(n, nz, th, 0.9)
+    i, j = np.mgrid[0:n, 0:n]
+    f = np.exp(-((i - n / 2 - 3) ** 2 + (j - n / 2 + 4) ** 2) / (2 * 5.0 ** 2))
+    obj = np.stack([f, 0.5 * f.T]) * (1 + 0.1j)
+    d = o.R(obj) + 0.01 * (rng.standard_normal((ntheta, nz, n)) + 1j * rng.standard_normal((ntheta, nz, n)))
+    d32 = d.astype(np.complex64)
+    u = g.rec_tomo(dev(d32), 8)
+    ref = o.rec_tomo(d32.astype(np.complex128), 8)
+    err = rel(u.cpu().numpy(), ref)
+    print(f"[rec_tomo] 8 iterations: {err:.2e}", flush=True)
+    assert err < 1e-4
