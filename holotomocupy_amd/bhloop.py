@@ -168,8 +168,9 @@ class BHLoop:
         angles through the fused pipeline, then RT (gF4), the regulariser and
         the probe fit."""
         grads["prb"].zero_()
+        grad = self.gradients_fused if self.fused_available() else self.gradients_cascade
         for st, B in self._chunks():
-            self.gradients_fused(self._sl(vars, st, B), self._sl(grads, st, B), self.data[st:st + B], st)
+            grad(self._sl(vars, st, B), self._sl(grads, st, B), self.data[st:st + B], st)
         self.gF4(grads["obj"], grads["proj"])
         self.gradient_laplacian(grads["obj"], vars["obj"])
         self.gradient_prbfit(grads["prb"], vars["prb"])
@@ -177,7 +178,7 @@ class BHLoop:
     def hessian_dev(self, vars, grads, etas):
         """hessian (rec_mpi.py:235-248) as a float64 device scalar."""
         out = self._scalar()
-        fused = getattr(self, "fused_hessian", True)
+        fused = getattr(self, "fused_hessian", True) and self.fused_available()
         for st, B in self._chunks():
             g = self._sl(grads, st, B)
             e = g if grads is etas else self._sl(etas, st, B)
@@ -197,7 +198,7 @@ class BHLoop:
         """(hessian(vars, grads, etas), hessian(vars, etas, etas)) as a float64
         device tensor [2]: the beta step's two forms, whose data terms share
         one fused pass per chunk of angles."""
-        if not getattr(self, "fused_hessian", True):
+        if not (getattr(self, "fused_hessian", True) and self.fused_available()):
             return torch.cat([self.hessian_dev(vars, grads, etas), self.hessian_dev(vars, etas, etas)])
         out = torch.zeros(2, dtype=torch.float64, device=self.device)
         for st, B in self._chunks():

@@ -736,6 +736,10 @@ int htamd_bh_plan_create(const htamd_prop_plan* prop, const htamd_shift_plan* sh
     return fail(HTAMD_ERR_ARG, "bh_plan_create: propagator and shift plans disagree on the detector size");
   if ((shift->npsi % 2) || (shift->nzpsi % 2))
     return fail(HTAMD_ERR_UNSUPPORTED, "bh_plan_create: object grid sizes must be even");
+  if (!fft_size_supported(prop->n) || !fft_size_supported(prop->nz))
+    return fail(HTAMD_ERR_UNSUPPORTED,
+                "bh_plan_create: the fused passes take power-of-two detector sizes in [8, 2048]; other sizes run "
+                "on the per-operator kernels");
   auto* p = new htamd_bh_plan();
   p->prop = prop;
   p->shift = shift;

@@ -92,10 +92,10 @@ int htamd_prop_plan_create(int64_t n, int64_t nz, int64_t ndist, double waveleng
                            double voxelsize, const double* distance, htamd_prop_plan** plan) {
   if (!plan) return fail(HTAMD_ERR_ARG, "prop_plan_create: plan is NULL");
   *plan = nullptr;
-  if (!fft_size_supported(n) || !fft_size_supported(nz))
+  if (!(fft_size_supported(n) || fresnel_any_supported(n)) || !(fft_size_supported(nz) || fresnel_any_supported(nz)))
     return fail(HTAMD_ERR_UNSUPPORTED,
-                "prop_plan_create: n and nz must be powers of two in [8, 2048], got n=" +
-                    std::to_string(n) + " nz=" + std::to_string(nz));
+                "prop_plan_create: n and nz must be even sizes in [8, 8192] (powers of two up to 2048 on the "
+                "fastest kernels), got n=" + std::to_string(n) + " nz=" + std::to_string(nz));
   if (ndist < 1 || !distance) return fail(HTAMD_ERR_ARG, "prop_plan_create: need ndist >= 1 distances");
   if (!(voxelsize > 0.0) || !(wavelength > 0.0))
     return fail(HTAMD_ERR_ARG, "prop_plan_create: wavelength and voxelsize must be > 0");

@@ -40,14 +40,14 @@ constexpr int col_stride() {
 
 // In-register DFTs.  Forward: X[k] = sum_r a[r] exp(-2 pi i r k / R).
 template <bool INV>
-__device__ __forceinline__ void dft2(float2& a0, float2& a1) {
+__host__ __device__ __forceinline__ void dft2(float2& a0, float2& a1) {
   float2 t = a0;
   a0 = cadd(t, a1);
   a1 = csub(t, a1);
 }
 
 template <bool INV>
-__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+__host__ __device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
   float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
   float2 t2 = cadd(a1, a3), t3 = mul_mi<INV>(csub(a1, a3));
   a0 = cadd(t0, t2);
@@ -57,7 +57,7 @@ __device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2&
 }
 
 template <bool INV>
-__device__ __forceinline__ void dft8(float2* a) {
+__host__ __device__ __forceinline__ void dft8(float2* a) {
   constexpr float h = 0.70710678118654752440f;
   float2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
   float2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];

@@ -14,6 +14,9 @@
 // C_k = (l_k - l_{n-k}) e^{-i pi k/n}/2n, tables built in FP64 on the host.
 // DT uses the tables of conj(lambda).  Each axis is one HBM pass: read the
 // field, FFT in LDS, pair map, inverse FFT in LDS, write the field.
+#include <mutex>
+
+#include "fft_any.hpp"
 #include "fresnel_core.hpp"
 #include "htamd_internal.hpp"
 
@@ -84,6 +87,87 @@ bool fft_size_supported(int64_t n) {
   return n >= kFftMin && n <= kFftMax && (n & (n - 1)) == 0;
 }
 
+bool fresnel_any_supported(int64_t n) {
+  AnyFft f;
+  return n >= kFftMin && n <= kAnyFresnelMax && n % 2 == 0 && any_fft_factor(n, &f) &&
+         (int64_t)2 * n * (int64_t)sizeof(float2) <= any_max_lds();
+}
+
+int64_t any_max_lds() {
+  static std::mutex mu;
+  static int64_t cache[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 65536;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!cache[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) v = 65536;
+    cache[dev] = v;
+  }
+  return cache[dev];
+}
+
+// ---- any even length (fft_any.hpp) ---------------------------------------------
+// Row pass: line = (field b, row y), slot p of the Makhoul-ordered line holds
+// element dct_nat(p); FFT, pair map, inverse FFT, written back in the same
+// order.  in may equal out (a line is read whole before it is written).
+struct FresnelRowsAny {
+  static constexpr bool kInv = false, kTwo = true;
+  const float2* in;
+  float2* out;
+  int64_t in_stride, out_stride;
+  int nz, n;
+  const float2* tab;
+  __device__ float2 load(int64_t line, int, int p) const {
+    return in[(line / nz) * in_stride + (line % nz) * n + dct_nat_rt(p, n)];
+  }
+  __device__ void mid(float2* s, int64_t, int, int tid, int nthr) const { dct_pair_map_rt(s, n, tab, tid, nthr); }
+  __device__ void store(int64_t line, int, int p, float2 v) const {
+    out[(line / nz) * out_stride + (line % nz) * n + dct_nat_rt(p, n)] = v;
+  }
+};
+
+// Column pass in place: unit = (field b, group of C columns from x0).
+struct FresnelColsAny {
+  static constexpr bool kInv = false, kTwo = true;
+  float2* data;
+  int64_t stride;
+  int nz, n, C, groups;
+  const float2* tab;
+  __device__ int64_t off(int64_t unit, int c, int p) const {
+    return (unit / groups) * stride + (int64_t)dct_nat_rt(p, nz) * n + (unit % groups) * C + c;
+  }
+  __device__ bool ok(int64_t unit, int c) const { return (unit % groups) * C + c < n; }
+  __device__ float2 load(int64_t unit, int c, int p) const {
+    return ok(unit, c) ? data[off(unit, c, p)] : make_float2(0.f, 0.f);
+  }
+  __device__ void mid(float2* s, int64_t, int, int tid, int nthr) const { dct_pair_map_rt(s, nz, tab, tid, nthr); }
+  __device__ void store(int64_t unit, int c, int p, float2 v) const {
+    if (ok(unit, c)) data[off(unit, c, p)] = v;
+  }
+};
+
+static hipError_t fresnel_rows_any(int64_t n, int64_t nz, const float2* in, int64_t in_stride, float2* out,
+                                   int64_t out_stride, int64_t batch, const float2* tw, const float2* tab,
+                                   hipStream_t st) {
+  AnyFft f;
+  if (!any_fft_factor(n, &f)) return hipErrorInvalidValue;
+  f.tw = tw;
+  const FresnelRowsAny op{in, out, in_stride, out_stride, (int)nz, (int)n, tab};
+  return any_launch_rows(op, f, batch * nz, st);
+}
+
+static hipError_t fresnel_cols_any(int64_t nz, int64_t n, float2* data, int64_t stride, int64_t batch,
+                                   const float2* tw, const float2* tab, hipStream_t st) {
+  AnyFft f;
+  if (!any_fft_factor(nz, &f)) return hipErrorInvalidValue;
+  f.tw = tw;
+  const int C = any_cols_c((int)nz, any_max_lds());
+  const int groups = (int)((n + C - 1) / C);
+  const FresnelColsAny op{data, stride, (int)nz, (int)n, C, groups, tab};
+  return any_launch_cols(op, f, C, batch * groups, st);
+}
+
 template <int N>
 static hipError_t rows_n(const float2* in, int64_t in_stride, float2* out, int64_t out_stride,
                          int64_t nz, int64_t batch, const float2* tw, const float2* tab,
@@ -101,6 +185,7 @@ static hipError_t rows_n(const float2* in, int64_t in_stride, float2* out, int64
 hipError_t launch_fresnel_rows(int64_t n, int64_t nz, const float2* in, int64_t in_stride,
                                float2* out, int64_t out_stride, int64_t batch, const float2* tw,
                                const float2* tab, hipStream_t st) {
+  if (!fft_size_supported(n)) return fresnel_rows_any(n, nz, in, in_stride, out, out_stride, batch, tw, tab, st);
 #define HTAMD_ROWS_CASE(NN) \
   case NN: return rows_n<NN>(in, in_stride, out, out_stride, nz, batch, tw, tab, st);
   switch (n) {
@@ -147,6 +232,7 @@ static hipError_t cols_n(int64_t n, float2* data, int64_t stride, int64_t batch,
 
 hipError_t launch_fresnel_cols(int64_t nz, int64_t n, float2* data, int64_t stride, int64_t batch,
                                const float2* tw, const float2* tab, hipStream_t st) {
+  if (!fft_size_supported(nz)) return fresnel_cols_any(nz, n, data, stride, batch, tw, tab, st);
 #define HTAMD_COLS_CASE(NN) \
   case NN: return cols_n<NN>(n, data, stride, batch, tw, tab, st);
   switch (nz) {

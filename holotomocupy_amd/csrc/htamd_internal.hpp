@@ -20,6 +20,11 @@ hipError_t upload_twiddles(float2** dst, int64_t N);
 constexpr int64_t kFftMin = 8;
 constexpr int64_t kFftMax = 2048;
 bool fft_size_supported(int64_t n);
+// Other even lengths in [kFftMin, kAnyFresnelMax] whose line fits the LDS run
+// on the run-time mixed-radix FFT (fft_any.hpp), slower than the power-of-two
+// kernels.
+constexpr int64_t kAnyFresnelMax = 8192;
+bool fresnel_any_supported(int64_t n);
 
 // Row pass over `batch` fields of nz rows of length n; field b starts at
 // in + b*in_stride / out + b*out_stride (elements); in may equal out.

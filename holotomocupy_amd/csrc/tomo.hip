@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/htamd.h"
+#include "fft_any.hpp"
 #include "fft_lds.hpp"
 #include "htamd_bspline.hpp"
 #include "htamd_internal.hpp"
@@ -440,6 +441,165 @@ __global__ __launch_bounds__(256) void tomo_rows_adj_k(TomoArgs a, TO* __restric
   }
 }
 
+// ---- any even n: the same passes on the run-time mixed-radix FFT --------------
+// (fft_any.hpp; rows of the embedded slice and lines are one unit each,
+// columns of the 2n grid C per unit).  Loads and stores apply the same sign
+// modulations, embedding, cropping and deapodisation as the kernels above.
+template <typename TI>
+struct TomoRowsFwdAny {  // line = z n + i' of the n non-zero rows, length 2n
+  static constexpr bool kInv = false, kTwo = false;
+  TomoArgs a;
+  const TI* obj;
+  __device__ float2 load(int64_t line, int, int p) const {
+    const int n = a.n, h = n / 2;
+    if (p < h || p >= h + n) return make_float2(0.f, 0.f);
+    const int ip = (int)(line % n), jp = p - h;
+    return cscale(ld_c<TI>(obj + line * (int64_t)n, jp), a.phi[ip * n + jp] * sgnf(ip + h + p));
+  }
+  __device__ void mid(float2*, int64_t, int, int, int) const {}
+  __device__ void store(int64_t line, int, int k, float2 v) const {
+    a.W[line * (int64_t)(2 * a.n) + k] = cscale(v, sgnf(k));
+  }
+};
+
+struct TomoColsFwdAny {  // unit = z groups + column group
+  static constexpr bool kInv = false, kTwo = false;
+  TomoArgs a;
+  int C, groups;
+  __device__ float2 load(int64_t unit, int c, int i) const {
+    const int n = a.n, h = n / 2, N2 = 2 * n;
+    const int col = (int)(unit % groups) * C + c;
+    if (col >= N2 || i < h || i >= h + n) return make_float2(0.f, 0.f);
+    return a.W[(unit / groups) * (int64_t)n * N2 + (int64_t)(i - h) * N2 + col];
+  }
+  __device__ void mid(float2*, int64_t, int, int, int) const {}
+  __device__ void store(int64_t unit, int c, int k, float2 v) const {
+    const int N2 = 2 * a.n;
+    const int col = (int)(unit % groups) * C + c;
+    if (col < N2) a.G[(unit / groups) * (int64_t)N2 * N2 + (int64_t)k * N2 + col] = cscale(v, sgnf(k));
+  }
+};
+
+template <typename TO>
+struct TomoLineIfftAny {  // line = theta nz + z, length n
+  static constexpr bool kInv = true, kTwo = false;
+  TomoArgs a;
+  TO* out;
+  __device__ float2 load(int64_t line, int, int t) const { return a.Sh[line * a.n + t]; }
+  __device__ void mid(float2*, int64_t, int, int, int) const {}
+  __device__ void store(int64_t line, int, int t, float2 v) const {
+    st_c<TO>(out + line * a.n, t, cscale(v, -sgnf(t) / (4.0f * a.n)));
+  }
+};
+
+template <typename TI>
+struct TomoLineFftAny {  // line = theta nz + z, length n; z-contiguous output
+  static constexpr bool kInv = false, kTwo = false;
+  TomoArgs a;
+  const TI* data;
+  __device__ float2 load(int64_t line, int, int t) const {
+    return cscale(ld_c<TI>(data + line * a.n, t), -sgnf(t));
+  }
+  __device__ void mid(float2*, int64_t, int, int, int) const {}
+  __device__ void store(int64_t line, int, int t, float2 v) const {
+    const int64_t th = line / a.nz, z = line % a.nz;
+    const float sc = a.filt ? -sgnf(t) * a.filt[t] : -sgnf(t);
+    a.Sh[(th * a.n + t) * a.nz + z] = cscale(v, sc);
+  }
+};
+
+struct TomoColsAdjAny {  // unit = z groups + column group; keeps the n central rows
+  static constexpr bool kInv = true, kTwo = false;
+  TomoArgs a;
+  int C, groups;
+  __device__ float2 load(int64_t unit, int c, int k) const {
+    const int N2 = 2 * a.n;
+    const int col = (int)(unit % groups) * C + c;
+    if (col >= N2) return make_float2(0.f, 0.f);
+    return cscale(a.G[(unit / groups) * (int64_t)N2 * N2 + (int64_t)k * N2 + col], sgnf(k + col));
+  }
+  __device__ void mid(float2*, int64_t, int, int, int) const {}
+  __device__ void store(int64_t unit, int c, int i, float2 v) const {
+    const int n = a.n, h = n / 2, N2 = 2 * n;
+    const int col = (int)(unit % groups) * C + c;
+    if (col < N2 && i >= h && i < h + n)
+      a.W[(unit / groups) * (int64_t)n * N2 + (int64_t)(i - h) * N2 + col] = cscale(v, sgnf(i) / (float)N2);
+  }
+};
+
+template <typename TO>
+struct TomoRowsAdjAny {  // line = z n + i', length 2n; crop and deapodise
+  static constexpr bool kInv = true, kTwo = false;
+  TomoArgs a;
+  TO* out;
+  __device__ float2 load(int64_t line, int, int k) const { return a.W[line * (int64_t)(2 * a.n) + k]; }
+  __device__ void mid(float2*, int64_t, int, int, int) const {}
+  __device__ void store(int64_t line, int, int p, float2 v) const {
+    const int n = a.n, h = n / 2;
+    if (p < h || p >= h + n) return;
+    const int ip = (int)(line % n), jp = p - h;
+    st_c<TO>(out + line * (int64_t)n, jp, cscale(v, sgnf(p) / (float)(2 * n) * a.phi[ip * n + jp]));
+  }
+};
+
+template <typename TIO>
+struct TomoFilterAny {  // IFFT(h FFT(line)), h with the 1/n of the inverse
+  static constexpr bool kInv = false, kTwo = true;
+  const TIO* in;
+  TIO* out;
+  int n;
+  const float* h;
+  __device__ float2 load(int64_t line, int, int k) const { return ld_c<TIO>(in + line * n, k); }
+  __device__ void mid(float2* s, int64_t, int, int tid, int nthr) const {
+    for (int k = tid; k < n; k += nthr) s[k] = cscale(s[k], h[k]);
+  }
+  __device__ void store(int64_t line, int, int k, float2 v) const { st_c<TIO>(out + line * n, k, v); }
+};
+
+static bool tomo_any_ffts(const TomoArgs& a, AnyFft* f2, AnyFft* f1) {
+  if (!any_fft_factor(2 * (int64_t)a.n, f2) || !any_fft_factor(a.n, f1)) return false;
+  f2->tw = a.tw2;
+  f1->tw = a.tw1;
+  return true;
+}
+
+template <typename TIO>
+static hipError_t tomo_fwd_any(const TomoArgs& a, const TIO* obj, TIO* out, hipStream_t st) {
+  AnyFft f2, f1;
+  if (!tomo_any_ffts(a, &f2, &f1) || !a.items) return hipErrorInvalidValue;
+  const int N2 = 2 * a.n;
+  hipError_t e = any_launch_rows(TomoRowsFwdAny<TIO>{a, obj}, f2, (int64_t)a.nz * a.n, st);
+  if (e != hipSuccess) return e;
+  const int C = any_cols_c(N2, any_max_lds()), groups = (N2 + C - 1) / C;
+  if ((e = any_launch_cols(TomoColsFwdAny{a, C, groups}, f2, C, (int64_t)a.nz * groups, st)) != hipSuccess) return e;
+  hipLaunchKernelGGL(tomo_tile_gather_k, dim3((unsigned)a.nitems, (unsigned)((a.nz + kTZ - 1) / kTZ)), dim3(256), 0,
+                     st, a);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  return any_launch_rows(TomoLineIfftAny<TIO>{a, out}, f1, (int64_t)a.ntheta * a.nz, st);
+}
+
+template <typename TIO>
+static hipError_t tomo_adj_any(const TomoArgs& a, const TIO* data, TIO* out, hipStream_t st) {
+  AnyFft f2, f1;
+  if (!tomo_any_ffts(a, &f2, &f1)) return hipErrorInvalidValue;
+  const int N2 = 2 * a.n;
+  hipError_t e = any_launch_rows(TomoLineFftAny<TIO>{a, data}, f1, (int64_t)a.ntheta * a.nz, st);
+  if (e != hipSuccess) return e;
+  const unsigned gx = (unsigned)(((int64_t)N2 * N2 + 255) / 256);
+  if (a.nz % 32 == 0)
+    hipLaunchKernelGGL((tomo_grid_gather_k<32, true>), dim3(gx, (unsigned)(a.nz / 32)), dim3(256), 0, st, a);
+  else if (a.nz % 16 == 0)
+    hipLaunchKernelGGL((tomo_grid_gather_k<16, true>), dim3(gx, (unsigned)(a.nz / 16)), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((tomo_grid_gather_k<16, false>), dim3(gx, (unsigned)((a.nz + 15) / 16)), dim3(256), 0, st, a);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const int C = any_cols_c(N2, any_max_lds()), groups = (N2 + C - 1) / C;
+  if ((e = any_launch_cols(TomoColsAdjAny{a, C, groups}, f2, C, (int64_t)a.nz * groups, st)) != hipSuccess) return e;
+  return any_launch_rows(TomoRowsAdjAny<TIO>{a, out}, f2, (int64_t)a.nz * a.n, st);
+}
+
+static bool tomo_pow2(int64_t n) { return n >= 8 && n <= 1024 && (n & (n - 1)) == 0; }
+
 // ---- host dispatch -----------------------------------------------------------
 // columns per workgroup of the column passes: at most 16, at most about 70 KiB
 // of LDS (8192 / N2 padded columns) and at most 1024 threads
@@ -521,6 +681,12 @@ static hipError_t tomo_filter_n(const TIO* in, TIO* out, int64_t lines, const fl
 template <typename TIO>
 static hipError_t tomo_filter_dispatch(int n, const void* in, void* out, int64_t lines, const float2* tw,
                                        const float* h, hipStream_t st) {
+  if (!tomo_pow2(n)) {
+    AnyFft f;
+    if (!any_fft_factor(n, &f)) return hipErrorInvalidValue;
+    f.tw = tw;
+    return any_launch_rows(TomoFilterAny<TIO>{(const TIO*)in, (TIO*)out, n, h}, f, lines, st);
+  }
 #define HTAMD_TFILT_CASE(NN) \
   case NN: return tomo_filter_n<NN, TIO>((const TIO*)in, (TIO*)out, lines, tw, h, st);
   switch (n) {
@@ -533,6 +699,9 @@ static hipError_t tomo_filter_dispatch(int n, const void* in, void* out, int64_t
 
 template <typename TIO>
 static hipError_t tomo_dispatch(bool adjoint, const TomoArgs& a, const void* in, void* out, hipStream_t st) {
+  if (!tomo_pow2(a.n))
+    return adjoint ? tomo_adj_any<TIO>(a, (const TIO*)in, (TIO*)out, st)
+                   : tomo_fwd_any<TIO>(a, (const TIO*)in, (TIO*)out, st);
 #define HTAMD_TOMO_CASE(NN)                                                               \
   case NN:                                                                                \
     return adjoint ? tomo_adj_n<2 * NN, TIO>(a, (const TIO*)in, (TIO*)out, st)            \
@@ -575,6 +744,9 @@ struct htamd_tomo_plan {
 namespace htamd {
 
 constexpr int kTomoFilters = 3;
+// largest slice width: the 2n-point lines of the oversampled grid (two LDS
+// buffers of 2n complex values) fit the 160 KiB of LDS up to n = 4096
+constexpr int64_t kTomoAnyMax = 4096;
 
 // h[k] on fftfreq(n) (src/holotomocupy/tomo.py:131-155), FP64:
 // ramp 4n|f|, Shepp-Logan 4n|f| sinc(f), Parzen 4n|f| w(u), u = 2 (4n|f|),
@@ -596,8 +768,15 @@ int htamd_tomo_plan_create(int64_t n, int64_t ntheta, const double* theta, const
                            htamd_tomo_plan** plan) {
   if (!plan || !theta) return fail(HTAMD_ERR_ARG, "tomo_plan_create: NULL argument");
   *plan = nullptr;
-  if (n < 8 || n > 1024 || (n & (n - 1)))
-    return fail(HTAMD_ERR_UNSUPPORTED, "tomo_plan_create: n must be a power of two in [8, 1024]");
+  {
+    AnyFft f2, f1;
+    const bool any_ok = n % 2 == 0 && n >= 8 && n <= kTomoAnyMax && any_fft_factor(2 * n, &f2) &&
+                        any_fft_factor(n, &f1);
+    if (!tomo_pow2(n) && !any_ok)
+      return fail(HTAMD_ERR_UNSUPPORTED, "tomo_plan_create: n must be even in [8, " + std::to_string(kTomoAnyMax) +
+                                             "] (powers of two up to 1024 on the fastest kernels), got " +
+                                             std::to_string(n));
+  }
   if (ntheta < 1) return fail(HTAMD_ERR_ARG, "tomo_plan_create: need ntheta >= 1");
   auto* p = new htamd_tomo_plan();
   p->n = n;
@@ -669,7 +848,7 @@ int htamd_tomo_plan_create(int64_t n, int64_t ntheta, const double* theta, const
   // first tap (wrapped), sorted by tile (counting sort, samples in increasing
   // order within a tile), cut into work items of at most 256 samples.
   const int ts = (int)std::min<int64_t>(32, n2);
-  const int ntx = (int)(n2 / ts);
+  const int ntx = (int)((n2 + ts - 1) / ts);  // the last tile is partial when ts does not divide 2n
   std::vector<int> tcount((size_t)(ntx * ntx + 1), 0);
   for (int64_t si = 0; si < S; ++si)
     ++tcount[(size_t)((wrap(tb[(size_t)si].y) / ts) * ntx + wrap(tb[(size_t)si].x) / ts + 1)];

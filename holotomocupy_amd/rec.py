@@ -302,6 +302,21 @@ class Rec(BHLoop):
         grads["proj"][:] = y[1] * self.rho_sq["obj"]
         grads["pos"][:] = y[2] * self.rho_sq["pos"]
 
+    def fused_available(self):
+        """True when the fused passes take this geometry: power-of-two detector
+        sizes up to 2048.  Other sizes (any even size up to 8192) run on the
+        per-operator kernels, which the BH loop then uses instead."""
+        if getattr(self, "_bh_ok", None) is None:
+            from .bh import BHGradient
+            try:
+                self._bh = BHGradient(self)
+                self._bh_ok = True
+            except _lib.HtamdError as e:
+                if "(-2)" not in str(e):
+                    raise
+                self._bh_ok = False
+        return self._bh_ok
+
     def gradients_fused(self, vars, grads, data=None, st=0, objective=None):
         """Same result as gradients_cascade through the fused three-pass HIP
         pipeline (holotomocupy_amd.bh); grads['prb'] is accumulated into.

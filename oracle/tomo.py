@@ -102,7 +102,9 @@ class Tomo:
     def __init__(self, n, nz, theta, mask_r):
         self.n = n
         self.nz = nz
-        self.theta = np.asarray(theta, dtype=np.float64)
+        # the reference keeps the angles as float32 (tomo.py:32); the taps are
+        # then evaluated in FP64 at those angles
+        self.theta = np.asarray(theta, dtype=np.float64).astype(np.float32).astype(np.float64)
         self.ntheta = len(self.theta)
         self.phi = deapodizer(n, self.ntheta, mask_r)
         self.c1d, self.c2d = signs(n)

@@ -50,7 +50,15 @@ def test_abi_version_and_error_path(lib):
     import numpy as np
     d = np.array([1.0])
     from holotomocupy_amd import _lib
-    rc = lib.htamd_prop_plan_create(12, 8, 1, 1.0, 1.0, d.ctypes.data_as(_lib.c_dp), ctypes.byref(plan))
-    assert rc == -2 and b"powers of two" in lib.htamd_last_error()
+    rc = lib.htamd_prop_plan_create(13, 8, 1, 1.0, 1.0, d.ctypes.data_as(_lib.c_dp), ctypes.byref(plan))
+    assert rc == -2 and b"even sizes" in lib.htamd_last_error()
+    rc = lib.htamd_prop_plan_create(16, 8200, 1, 1.0, 1.0, d.ctypes.data_as(_lib.c_dp), ctypes.byref(plan))
+    assert rc == -2
+    theta = np.linspace(0, np.pi, 8, endpoint=False)
+    rc = lib.htamd_tomo_plan_create(13, 8, theta.ctypes.data_as(_lib.c_dp), None, ctypes.byref(plan))
+    assert rc == -2 and b"even" in lib.htamd_last_error()
+    rc = lib.htamd_tomo_plan_create(4098, 8, theta.ctypes.data_as(_lib.c_dp), None, ctypes.byref(plan))
+    assert rc == -2
     rc = lib.htamd_shift_plan_create(0, 8, 8, 8, 1, ctypes.byref(plan))
     assert rc == -1
+
