@@ -93,11 +93,16 @@ def test_served_mix_against_oracle(gpu):
     p2, j2, s2 = host_inputs(rng, G1024, 2)
     cas2, _ = oracle_cascade(G1024, 2, 720 * 3 * 1024 * 1024, rho)
     d2 = amplitudes(cas2, p2, j2, s2, rng)
-    x2 = [(p2 * 0.9).astype(np.complex64), (j2 * 0.8).astype(np.complex64), s2]
+    # positions moved off the truth as for 2048^2: at the true positions the
+    # residual is the Poisson noise alone and the position gradient is a sum
+    # of nearly cancelling terms, whose FP32 rounding does not average out
+    x2 = [(p2 * 0.9).astype(np.complex64), (j2 * 0.8).astype(np.complex64),
+          (s2 + rng.uniform(-0.3, 0.3, s2.shape)).astype(np.float32)]
     p3, j3, s3 = host_inputs(rng, G256, 8)
     cas3, _ = oracle_cascade(G256, 8, 8 * 2 * 256 * 256, rho)
     d3 = amplitudes(cas3, p3, j3, s3, rng)
-    x3 = [(p3 * 1.1).astype(np.complex64), (j3 * 0.7).astype(np.complex64), s3]
+    x3 = [(p3 * 1.1).astype(np.complex64), (j3 * 0.7).astype(np.complex64),
+          (s3 + rng.uniform(-0.3, 0.3, s3.shape)).astype(np.float32)]
     # bilinear form at 512^2 (2 distances, 2 angles)
     p4, j4, s4 = host_inputs(rng, G512, 2)
     cas4, _ = oracle_cascade(G512, 2, 2 * 2 * 512 * 512, (1.0, 1.0, 1.0))

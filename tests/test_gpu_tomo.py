@@ -121,7 +121,9 @@ def test_rejects_bad_sizes(gpu):
     from holotomocupy_amd import Tomo
     from holotomocupy_amd._lib import HtamdError
     with pytest.raises(HtamdError):
-        Tomo(48, 1, angles(4), 0)
+        Tomo(47, 1, angles(4), 0)  # odd widths are refused (any even width up to 4096 is taken)
+    with pytest.raises(HtamdError):
+        Tomo(4098, 1, angles(4), 0)
     g = Tomo(16, 2, angles(4), 0)
     with pytest.raises(ValueError):
         g.R(torch.zeros((3, 16, 16), device="cuda"))
@@ -201,12 +203,16 @@ def test_filter_and_fbp_match_oracle(gpu, name, n, ntheta, nz, cplx):
 
 
 def test_rec_tomo_matches_oracle(gpu):
-    """Eight CG iterations of rec_tomo (tomo.py:188-210) against the FP64
-    oracle's: relative L2 <= 1e-4 (each iteration adds the FP32 error of one
-    R and one RT, ~5e-6 each, and alpha / beta are ratios of such sums)."""
+    """CG iterations of rec_tomo (tomo.py:188-210) against the FP64 oracle's.
+    After four iterations the iterate is compared at 1e-4 relative L2 (each
+    iteration adds the FP32 error of one R and one RT, ~5e-6 each).  Later
+    iterations are ill-conditioned: on the CPU, perturbing the oracle's R and
+    RT by 5e-6 relative noise moves its 8th iterate by 4e-2 and its objective
+    by 4e-3, so after eight iterations the test checks the objective
+    |R u - d|^2: decreasing every iteration and within 1e-2 of the oracle's."""
     from holotomocupy_amd import Tomo
     from oracle import tomo as ot
-    print("[case] rec_tomo 8 iterations n=64", flush=True)
+    print("[case] rec_tomo 4 / 8 iterations n=64", flush=True)
     n, ntheta, nz = 64, 96, 2
     th = angles(ntheta)
     rng = np.random.default_rng(9)
@@ -216,8 +222,16 @@ def test_rec_tomo_matches_oracle(gpu):
     obj = np.stack([f, 0.5 * f.T]) * (1 + 0.1j)
     d = o.R(obj) + 0.01 * (rng.standard_normal((ntheta, nz, n)) + 1j * rng.standard_normal((ntheta, nz, n)))
     d32 = d.astype(np.complex64)
-    u = g.rec_tomo(dev(d32), 8)
-    ref = o.rec_tomo(d32.astype(np.complex128), 8)
-    err = rel(u.cpu().numpy(), ref)
-    print(f"[rec_tomo] 8 iterations: {err:.2e}", flush=True)
+    d64 = d32.astype(np.complex128)
+    u4 = g.rec_tomo(dev(d32), 4)
+    err = rel(u4.cpu().numpy(), o.rec_tomo(d64, 4))
+    print(f"[rec_tomo] 4 iterations: {err:.2e}", flush=True)
     assert err < 1e-4
+    objs = []
+    for it in range(1, 9):
+        u = g.rec_tomo(dev(d32), it).cpu().numpy().astype(np.complex128)
+        objs.append(np.linalg.norm(o.R(u) - d64) ** 2)
+    assert all(b < a for a, b in zip(objs, objs[1:]))
+    ref8 = np.linalg.norm(o.R(o.rec_tomo(d64, 8)) - d64) ** 2
+    print(f"[rec_tomo] objective after 8 iterations: {objs[-1]:.6e} (oracle {ref8:.6e})", flush=True)
+    assert abs(objs[-1] - ref8) / ref8 < 1e-2
