@@ -66,6 +66,8 @@ SIGNATURES = {
     "htamd_bspline_d1_adj_workspace_size": [c_vp, c_i64, ctypes.POINTER(c_i64)],
     "htamd_bspline_d1_adj": [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64,
                              c_i64, c_vp],
+    "htamd_bspline_d1_adj_noalloc_max_bytes": [ctypes.POINTER(c_i64)],
+    "htamd_bspline_d1_adj_noalloc": [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp],
 }
 
 _lib = None

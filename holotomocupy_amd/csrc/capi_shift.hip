@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -266,6 +267,76 @@ int htamd_bspline_d1_adj(const htamd_shift_plan* p, const void* c, const void* g
     }
   }
   return e == hipSuccess ? HTAMD_OK : hip_fail(e, "bspline_d1_adj");
+}
+
+}  // extern "C"
+
+// ---- allocation-free variant of htamd_bspline_d1_adj -------------------------
+// The per-tile partial sums live in a file-scope device array instead of a
+// caller workspace.  Host code never passes `scratch` itself to a kernel (on
+// the host that name is only a stand-in); it passes the device address that
+// hipGetSymbolAddress returns, looked up once per device and kept here.
+namespace {
+
+constexpr int64_t kD1AdjScratchBytes = int64_t(4) << 20;  // 4 MiB: 128 angles of 2048^2
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kD1AdjScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// *out = device address of the current device's copy of `scratch`; the
+// lookup runs once per device
+int scratch_on_current_device(const char* what, void** out) {
+  *out = nullptr;
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, what);
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(HTAMD_ERR_UNSUPPORTED, std::string(what) + ": device index " + std::to_string(dev) + " out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, what);
+    if (!p) return fail(HTAMD_ERR_HIP, std::string(what) + ": hipGetSymbolAddress returned NULL");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return HTAMD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int htamd_bspline_d1_adj_noalloc_max_bytes(int64_t* bytes) {
+  if (!bytes) return fail(HTAMD_ERR_ARG, "bspline_d1_adj_noalloc_max_bytes: NULL");
+  *bytes = kD1AdjScratchBytes;
+  return HTAMD_OK;
+}
+
+int htamd_bspline_d1_adj_noalloc(const htamd_shift_plan* p, const void* c, const void* g, const float* r,
+                                 int64_t r_stride, const float* m, int64_t m_stride, void* out1, float* out2,
+                                 int64_t batch, void* stream) {
+  int s = check_common(p, batch, r, m, "bspline_d1_adj_noalloc");
+  if (s || batch == 0) return s;
+  if (!c || !g || !out1 || !out2) return fail(HTAMD_ERR_ARG, "bspline_d1_adj_noalloc: NULL buffer");
+  int64_t need = 0;
+  htamd_bspline_d1_adj_workspace_size(p, batch, &need);
+  if (need > kD1AdjScratchBytes)
+    return fail(HTAMD_ERR_UNSUPPORTED, "bspline_d1_adj_noalloc: " + std::to_string(batch) + " angles need " +
+                                           std::to_string(need) + " bytes of scratch, the fixed maximum is " +
+                                           std::to_string(kD1AdjScratchBytes) +
+                                           " (split the batch or use htamd_bspline_d1_adj)");
+  void* ws = nullptr;
+  s = scratch_on_current_device("bspline_d1_adj_noalloc", &ws);
+  if (s) return s;
+  // every partial is written by the first reduction kernel before the second
+  // reads it, so the scratch needs no clearing
+  return htamd_bspline_d1_adj(p, c, g, r, r_stride, m, m_stride, out1, out2, ws, kD1AdjScratchBytes, batch,
+                              stream);
 }
 
 }  // extern "C"

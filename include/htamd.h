@@ -136,6 +136,20 @@ int htamd_bspline_d1_adj(const htamd_shift_plan* plan, const void* c, const void
                          int64_t r_stride, const float* m, int64_t m_stride, void* out1, float* out2,
                          void* workspace, int64_t workspace_bytes, int64_t batch, void* stream);
 
+/* Allocation-free form of htamd_bspline_d1_adj.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The partial sums live in a fixed device array of the library, one per GPU,
+ * of htamd_bspline_d1_adj_noalloc_max_bytes() bytes (4 MiB: 128 angles of a
+ * 2048^2 detector); a batch that needs more returns HTAMD_ERR_UNSUPPORTED.  It
+ * calls no hipMalloc / hipFree.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (different streams or threads) race.  Results
+ * equal htamd_bspline_d1_adj's bit for bit. */
+int htamd_bspline_d1_adj_noalloc_max_bytes(int64_t* bytes);
+int htamd_bspline_d1_adj_noalloc(const htamd_shift_plan* plan, const void* c, const void* g, const float* r,
+                                 int64_t r_stride, const float* m, int64_t m_stride, void* out1, float* out2,
+                                 int64_t batch, void* stream);
+
 /* Magnified back-shift of the Paganin initial guess (complex64 only):
  * c_out[batch, nzpsi, npsi] = Sback(g, r, m), g [batch, nz, n], the stretched
  * spline phi(t/m) of support |t| < 2m, computed as a deterministic gather.

@@ -62,3 +62,23 @@ def test_abi_version_and_error_path(lib):
     rc = lib.htamd_shift_plan_create(0, 8, 8, 8, 1, ctypes.byref(plan))
     assert rc == -1
 
+
+def test_noalloc_d1_adj_rejects_batches_beyond_its_scratch(lib):
+    """The allocation-free dcurlySadjc refuses, on the host and before it looks
+    up its device scratch, a batch whose partial sums exceed the fixed maximum
+    (no GPU is touched: a 3000-wide plan uploads no FFT tables and the buffer
+    arguments are never dereferenced)."""
+    from holotomocupy_amd import _lib
+    cap = ctypes.c_int64()
+    assert lib.htamd_bspline_d1_adj_noalloc_max_bytes(ctypes.byref(cap)) == 0 and cap.value == 4 << 20
+    plan = ctypes.c_void_p()
+    assert lib.htamd_shift_plan_create(3000, 3000, 3000, 3000, 1, ctypes.byref(plan)) == 0
+    try:
+        per = ctypes.c_int64()
+        assert lib.htamd_bspline_d1_adj_workspace_size(plan, 1, ctypes.byref(per)) == 0
+        B = cap.value // per.value + 1
+        fake = _lib.c_vp(256)
+        rc = lib.htamd_bspline_d1_adj_noalloc(plan, fake, fake, fake, 2, fake, 1, fake, fake, B, None)
+        assert rc == -2 and b"fixed maximum" in lib.htamd_last_error()
+    finally:
+        lib.htamd_shift_plan_destroy(plan)
