@@ -228,12 +228,14 @@ def roofline_fused(a, rec, vars_, grads, data, steps=3):
             "pass_ms": {s: round(t, 3) for s, t in zip(STAGES, tot)}}
 
 
-def cpu_baseline(a, host, vars_, data):
+def cpu_baseline(a, host, vars_, data, runs=5):
     """Oracle (NumPy/SciPy port of the reference cascade) on one angle of the
     same workload: one BH gradient evaluation at the timed variables and data
-    (angle 0), timed once.  Returns the baseline record and (inputs, float32
-    data, FP64 gradients) for the parity check of the GPU path on the same
-    sample."""
+    (angle 0), in FP32 (complex64 transforms and sums, the reference's
+    precision) and in FP64, each timed as the median of `runs` runs after one
+    warm-up (the FP64 warm-up run also gives the reference gradient).  Returns
+    the baseline record and (inputs, float32 data, FP64 gradients) for the
+    parity check of the GPU path on the same sample."""
     import numpy as np
     from oracle import cascade as oc
     from oracle import propagation as op
@@ -244,22 +246,36 @@ def cpu_baseline(a, host, vars_, data):
     op.WORKERS = cores
     osh.WORKERS = cores
     ph = physics(g["energy"], g["z1"], g["focustodetectordistance"], g["detector_pixelsize"])
-    prop = op.Propagation(a.n, a.n, 1, a.ndist, ph.wavelength, ph.voxelsize, ph.distance)
-    sh = osh.Shift(a.n, a.nobj, a.n, a.nobj)
     mag = (1 / ph.norm_magnifications)[None, :]
-    cas = oc.Cascade(prop, sh, a.ndist, a.n, a.n, a.nobj, a.nobj, a.angles * a.ndist * a.n * a.n, mag,
-                     rho=tuple(args.rho))
     prb = vars_["prb"].cpu().numpy()
     proj = vars_["proj"][:1].cpu().numpy()
     pos = vars_["pos"][:1].cpu().numpy()
     d = data[:1].cpu().numpy()
-    x = [prb.astype(np.complex128), proj.astype(np.complex128), pos.astype(np.float64)]
-    t0 = time.perf_counter()
-    ref = cas.gradients(*x, d.astype(np.float64))
-    dt = time.perf_counter() - t0
-    base = {"value": round(1.0 / dt, 5), "unit": "evals/s", "cores": cores, "cpu_model": model, "kind": "port",
-            "sample": f"1 angle x {a.ndist} distances at {a.n}^2 / object {a.nobj}^2, FP64 NumPy/SciPy "
-                      f"oracle cascade (scipy.fft workers={cores}), one run of {dt:.1f} s"}
+    times, ref = {}, None
+    for single in (False, True):
+        cd = np.complex64 if single else np.complex128
+        prop = op.Propagation(a.n, a.n, 1, a.ndist, ph.wavelength, ph.voxelsize, ph.distance, dtype=cd)
+        cas = oc.Cascade(prop, osh.Shift(a.n, a.nobj, a.n, a.nobj, single=single), a.ndist, a.n, a.n, a.nobj,
+                         a.nobj, a.angles * a.ndist * a.n * a.n, mag, rho=tuple(args.rho), single=single)
+        x = [prb.astype(cd), proj.astype(cd), pos.astype(np.float64)]
+        dd = d.astype(np.float32 if single else np.float64)
+        ts = []
+        for k in range(runs + 1):
+            t0 = time.perf_counter()
+            out = cas.gradients(*x, dd)
+            if k:
+                ts.append(time.perf_counter() - t0)
+            elif not single:
+                ref = out
+            print(f"cpu_baseline: {'FP32' if single else 'FP64'} run {k + 1}/{runs + 1} done", file=sys.stderr,
+                  flush=True)
+        times["fp32" if single else "fp64"] = float(np.median(ts))
+    t32, t64 = times["fp32"], times["fp64"]
+    base = {"value": round(1.0 / t32, 5), "unit": "evals/s", "cores": cores, "cpu_model": model, "kind": "port",
+            "precision": "fp32", "fp64": {"value": round(1.0 / t64, 5), "unit": "evals/s", "median_s": round(t64, 3)},
+            "sample": f"1 angle x {a.ndist} distances at {a.n}^2 / object {a.nobj}^2, NumPy/SciPy oracle cascade "
+                      f"(scipy.fft workers={cores}), median of {runs} runs after 1 warm-up: FP32 {t32:.2f} s, "
+                      f"FP64 {t64:.2f} s"}
     return base, (prb, proj, pos, d, ref)
 
 
@@ -307,18 +323,36 @@ def measured_copy_gbs(nbytes=1 << 30, reps=10):
 
 
 def _host_cpu():
-    """(threads used by the oracle, CPU model) of this host."""
+    """(threads used by the oracle, CPU model) of this host: lscpu's model
+    name with its vendor family / model numbers (a virtual machine may report
+    only a generic name)."""
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     cores = max(1, min(cores, os.cpu_count() or 1))
-    model = "unknown"
+    fields = {}
     try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                fields[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
         pass
+    if "Model name" not in fields:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        fields["Model name"] = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    model = fields.get("Model name", "unknown")
+    ids = [f"{k.lower()} {fields[k]}" for k in ("CPU family", "Model", "Stepping") if k in fields]
+    if ids:
+        model += " (" + ", ".join(ids) + ")"
+    if "Socket(s)" in fields and "Core(s) per socket" in fields:
+        model += f", {fields['Socket(s)']} socket(s) x {fields['Core(s) per socket']} cores"
     return cores, model
 
 

@@ -217,7 +217,7 @@ __global__ __launch_bounds__(kRowThreads) void bh_fwd_rows_k(BhArgs a) {
 }
 
 // ---- K2: columns: D_y, residual, DT_y ---------------------------------------
-template <int N, int E, int C>
+template <int N, int E, int C, int RM = kMaxRadix, bool WAVE = false>
 __global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_field) {
   constexpr int T = N / E, NT = C * T, LS = col_stride<N, C>();
   __shared__ float2 lds[C * LS];
@@ -228,14 +228,14 @@ __global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_f
   const int64_t b = f / a.ndist;
   const int j = (int)(f % a.ndist);
   const int cc = threadIdx.x / T, t = threadIdx.x % T;
-  FftTw<N, E> twr;
-  fft_load_twiddles<N, E>(twr, t, a.twy);
+  FftTw<N, E, RM> twr;
+  fft_load_twiddles<N, E, RM>(twr, t, a.twy);
   float2* base = a.T + f * (int64_t)N * a.n + x0;
   const float* dbase = a.data + f * (int64_t)N * a.n + x0;
   col_tile_xfer<N, E, C, true>(lds, base, a.n, threadIdx.x);  // FFT slot q holds row dct_nat(q)
   __syncthreads();
   float2* s = lds + cc * LS;
-  fresnel_1d<N, E>(s, t, twr, a.taby + (int64_t)(j * 2 + 0) * 3 * (N / 2));
+  fresnel_1d<N, E, RM, WAVE>(s, t, twr, a.taby + (int64_t)(j * 2 + 0) * 3 * (N / 2));
   const float sc = a.scales[b].x;
   double obj = 0.0;
   int tr = t;
@@ -245,18 +245,25 @@ __global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_f
   float dv[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) dv[e] = dbase[dct_nat<N>(tr + e * T) * a.n + cc];
+  // |x| and d/|x| from one reciprocal square root (v_rsq_f32, ~1 ulp) instead
+  // of a correctly rounded square root and division; this thread's E terms of
+  // the objective are summed in FP32 and added to the FP64 total once
+  float objf = 0.0f;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int p = fpad(tr + e * T);
     const float2 v = s[p];
-    const float am = sqrtf(v.x * v.x + v.y * v.y);
-    const float q = am - dv[e];
-    obj += (double)(q * q);
-    const float g = sc * (1.0f - dv[e] / am);
+    const float a2 = v.x * v.x + v.y * v.y;
+    const float ri = __builtin_amdgcn_rsqf(a2);
+    const float q = a2 * ri - dv[e];
+    objf = fmaf(q, q, objf);
+    const float g = sc * (1.0f - dv[e] * ri);
     s[p] = make_float2(g * v.x, g * v.y);
   }
-  __syncthreads();
-  fresnel_1d<N, E>(s, t, twr, a.taby + (int64_t)(j * 2 + 1) * 3 * (N / 2));
+  obj += (double)objf;
+  lds_sync<WAVE>();
+  fresnel_1d<N, E, RM, WAVE>(s, t, twr, a.taby + (int64_t)(j * 2 + 1) * 3 * (N / 2));
+  if constexpr (WAVE) __syncthreads();  // the store reads every wave's column
   int tid = threadIdx.x;
   asm volatile("" : "+v"(tid));
   col_tile_xfer<N, E, C, false>(lds, base, a.n, tid);

@@ -148,35 +148,36 @@ __device__ __forceinline__ void dft(float2* a) {
 }
 
 // ---- compile-time stage schedule -------------------------------------------
-// Stage radix: at most 16 (in-register DFTs up to radix 16); a thread owning
-// E > 16 values performs E/R butterflies per stage.
+// Stage radix: at most RM (in-register DFTs up to radix 16; RM = 8 keeps the
+// butterfly temporaries in fewer registers); a thread owning E > RM values
+// performs E/R butterflies per stage.
 constexpr int kMaxRadix = 16;
-template <int N, int E, int NS>
+template <int N, int E, int NS, int RM = kMaxRadix>
 constexpr int stage_radix() {
-  constexpr int R0 = E < kMaxRadix ? E : kMaxRadix;
+  constexpr int R0 = E < RM ? E : RM;
   return (N / NS) >= R0 ? R0 : (N / NS);
 }
 // number of base twiddles a thread holds for stages starting at span NS
-template <int N, int E, int NS = 1>
+template <int N, int E, int NS = 1, int RM = kMaxRadix>
 constexpr int tw_count() {
   if constexpr (NS >= N) {
     return 0;
   } else {
-    constexpr int R = stage_radix<N, E, NS>();
-    return (NS > 1 ? E / R : 0) + tw_count<N, E, NS * R>();
+    constexpr int R = stage_radix<N, E, NS, RM>();
+    return (NS > 1 ? E / R : 0) + tw_count<N, E, NS * R, RM>();
   }
 }
 
-template <int N, int E>
+template <int N, int E, int RM = kMaxRadix>
 struct FftTw {
-  static constexpr int K = tw_count<N, E>() > 0 ? tw_count<N, E>() : 1;
+  static constexpr int K = tw_count<N, E, 1, RM>() > 0 ? tw_count<N, E, 1, RM>() : 1;
   float2 w[K];
 };
 
-template <int N, int E, int NS, int OFF>
+template <int N, int E, int NS, int OFF, int RM>
 __device__ __forceinline__ void load_tw_rec(float2* w, int t, const float2* __restrict__ tw) {
   if constexpr (NS < N) {
-    constexpr int R = stage_radix<N, E, NS>();
+    constexpr int R = stage_radix<N, E, NS, RM>();
     constexpr int T = N / E;
     if constexpr (NS > 1) {
 #pragma unroll
@@ -185,15 +186,30 @@ __device__ __forceinline__ void load_tw_rec(float2* w, int t, const float2* __re
         w[OFF + b] = tw[k * (N / (NS * R))];
       }
     }
-    load_tw_rec<N, E, NS * R, OFF + (NS > 1 ? E / R : 0)>(w, t, tw);
+    load_tw_rec<N, E, NS * R, OFF + (NS > 1 ? E / R : 0), RM>(w, t, tw);
   }
 }
 
 // Load this thread's base twiddles (t in [0, N/E)).  Issue early: the loads
 // overlap with whatever the kernel does before the first transform.
-template <int N, int E>
-__device__ __forceinline__ void fft_load_twiddles(FftTw<N, E>& r, int t, const float2* __restrict__ tw) {
-  load_tw_rec<N, E, 1, 0>(r.w, t, tw);
+template <int N, int E, int RM = kMaxRadix>
+__device__ __forceinline__ void fft_load_twiddles(FftTw<N, E, RM>& r, int t, const float2* __restrict__ tw) {
+  load_tw_rec<N, E, 1, 0, RM>(r.w, t, tw);
+}
+
+// Ordering of LDS accesses between the phases of a transform.  WAVE: every
+// thread that touches the buffer belongs to one wavefront, whose LDS
+// instructions complete in issue order, so keeping the compiler from moving
+// accesses across the point is enough; otherwise a workgroup barrier.
+template <bool WAVE>
+__device__ __forceinline__ void lds_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
 }
 
 // powers p[r] = w^r, r = 1..R-1 (p[0] unused), products of depth <= 4
@@ -219,7 +235,7 @@ __device__ __forceinline__ void tw_powers(float2 w, float2* p) {
 }
 
 // One Stockham stage: span NS (product of radices done so far), radix R.
-template <int N, int E, int R, int NS, bool INV>
+template <int N, int E, int R, int NS, bool INV, bool WAVE = false>
 __device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* w) {
   constexpr int T = N / E;
   constexpr int NB = E / R;
@@ -236,7 +252,7 @@ __device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* w
       for (int r = 0; r < R; ++r) a[b][r] = s[fpad(j + r * (N / R))];
     }
   }
-  __syncthreads();
+  lds_sync<WAVE>();
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const int j = t + b * T;
@@ -260,27 +276,29 @@ __device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* w
       for (int r = 0; r < R; ++r) s[fpad(base + r * NS)] = a[b][r];
     }
   }
-  __syncthreads();
+  lds_sync<WAVE>();
 }
 
-template <int N, int E, int NS, int OFF, bool INV>
+template <int N, int E, int NS, int OFF, bool INV, int RM, bool WAVE>
 __device__ __forceinline__ void stockham(float2* s, int t, const float2* w) {
   if constexpr (NS < N) {
-    constexpr int R = stage_radix<N, E, NS>();
-    stockham_stage<N, E, R, NS, INV>(s, t, w + OFF);
-    stockham<N, E, NS * R, OFF + (NS > 1 ? E / R : 0), INV>(s, t, w);
+    constexpr int R = stage_radix<N, E, NS, RM>();
+    stockham_stage<N, E, R, NS, INV, WAVE>(s, t, w + OFF);
+    stockham<N, E, NS * R, OFF + (NS > 1 ? E / R : 0), INV, RM, WAVE>(s, t, w);
   }
 }
 
 // Transform the buffer s (fft_buf<N>() slots, element i at s[fpad(i)]) in
-// place; every thread of the workgroup must call this
-// (it contains barriers).  t in [0, N/E) is the thread's index within this FFT
-// and tw its registers from fft_load_twiddles with the same t.
-template <int N, int E, bool INV>
-__device__ __forceinline__ void fft_lds(float2* s, int t, const FftTw<N, E>& tw) {
+// place; every thread of the workgroup must call this (it contains barriers;
+// with WAVE, every thread of the wavefront that owns the buffer, which must
+// be all N/E threads of the transform).  t in [0, N/E) is the thread's index
+// within this FFT and tw its registers from fft_load_twiddles with the same t.
+template <int N, int E, bool INV, int RM = kMaxRadix, bool WAVE = false>
+__device__ __forceinline__ void fft_lds(float2* s, int t, const FftTw<N, E, RM>& tw) {
   static_assert((N & (N - 1)) == 0 && N >= 2, "N must be a power of two");
   static_assert(E <= N && (E == 2 || E == 4 || E == 8 || E == 16 || E == 32), "E must be 2..32");
-  stockham<N, E, 1, 0, INV>(s, t, tw.w);
+  static_assert(!WAVE || N / E == 64, "a wavefront-private transform takes exactly 64 threads");
+  stockham<N, E, 1, 0, INV, RM, WAVE>(s, t, tw.w);
 }
 
 // Makhoul reordering that turns a length-N DCT-II into a length-N FFT:

@@ -14,7 +14,7 @@ namespace htamd {
 // Pair map on the FFT output held in LDS (see file comment).  Table layout:
 // tab[k] = A_k, tab[H+k] = B_k, tab[2H+k] = C_k (H = N/2), except that the two
 // self-paired bins use tab[0] = l_0/N and tab[H] = l_{N/2}/N.
-template <int N, int E>
+template <int N, int E, bool WAVE = false>
 __device__ __forceinline__ void dct_pair_map(float2* s, int t, const float2* __restrict__ tab) {
   constexpr int T = N / E;
   constexpr int H = N / 2;
@@ -34,15 +34,15 @@ __device__ __forceinline__ void dct_pair_map(float2* s, int t, const float2* __r
     // keep the compiler from hoisting all 3 E/2 table loads at once (registers)
     if (q % 4 == 3) asm volatile("" ::: "memory");
   }
-  __syncthreads();
+  lds_sync<WAVE>();
 }
 
-template <int N, int E>
-__device__ __forceinline__ void fresnel_1d(float2* s, int t, const FftTw<N, E>& tw,
+template <int N, int E, int RM = kMaxRadix, bool WAVE = false>
+__device__ __forceinline__ void fresnel_1d(float2* s, int t, const FftTw<N, E, RM>& tw,
                                            const float2* __restrict__ tab) {
-  fft_lds<N, E, false>(s, t, tw);
-  dct_pair_map<N, E>(s, t, tab);
-  fft_lds<N, E, true>(s, t, tw);
+  fft_lds<N, E, false, RM, WAVE>(s, t, tw);
+  dct_pair_map<N, E, WAVE>(s, t, tab);
+  fft_lds<N, E, true, RM, WAVE>(s, t, tw);
 }
 
 // Column-pass elements per thread: 32 at N = 2048 (64 threads per column, so a

@@ -14,6 +14,7 @@
 // taps transposed (one CSR row per grid point) and gathers onto the grid from
 // a slice-contiguous copy of the transformed sinogram -- the reference's atomic
 // scatter in a fixed summation order.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -51,6 +52,10 @@ struct TomoArgs {
   float2* W = nullptr;             // [nz, n, 2n]
   float2* G = nullptr;             // [nz, 2n, 2n]
   float2* Sh = nullptr;            // [ntheta n, nz] transformed sinogram (RT)
+  // grid points whose CSR rows are longer than kHeavyRow, longest first; the
+  // thread-per-point gather skips them and a wavefront per point sums them
+  const int* heavy = nullptr;
+  int nheavy = 0;
   // per-bin factor of the adjoint's line transform (NULL = none): the
   // detector filter of filtered back-projection, shifted by n/2 to the bins
   // of the c1d-modulated transform and scaled by sqrt(n / ntheta)
@@ -340,16 +345,23 @@ __global__ __launch_bounds__(256) void tomo_filter_k(const TIO* in, TIO* out, in
 // grid point are summed in sample order (deterministic, no atomics).  Each
 // CSR entry is one 8-byte (sample, weight) pair, and ZB = 32 where the slice
 // count allows, so the index and weight loads are amortised over more slices.
+// Rows through the centre of the grid collect a sample of every angle: at
+// 256^2 with 384 angles the longest has 2005 entries against a mean of 30, and
+// the one thread that owned it set the kernel's duration.  Rows longer than
+// kHeavyRow entries go to tomo_grid_gather_heavy_k instead.
+constexpr int kHeavyRow = 64;
+
 template <int ZB, bool FULL>
 __global__ __launch_bounds__(256) void tomo_grid_gather_k(TomoArgs a) {
   const int64_t n4 = 4LL * a.n * a.n;
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n4) return;
+  const int k0 = a.rp[g], k1 = a.rp[g + 1];
+  if (k1 - k0 > kHeavyRow) return;  // summed by tomo_grid_gather_heavy_k
   const int z0 = blockIdx.y * ZB;
   float2 acc[ZB];
 #pragma unroll
   for (int q = 0; q < ZB; ++q) acc[q] = make_float2(0.f, 0.f);
-  const int k0 = a.rp[g], k1 = a.rp[g + 1];
   for (int k = k0; k < k1; ++k) {
     const int2 cw = a.csw[k];
     const float w = __int_as_float(cw.y);
@@ -371,6 +383,76 @@ __global__ __launch_bounds__(256) void tomo_grid_gather_k(TomoArgs a) {
 #pragma unroll
   for (int q = 0; q < ZB; ++q)
     if (FULL || z0 + q < a.nz) a.G[(int64_t)(z0 + q) * n4 + g] = acc[q];
+}
+
+// One wavefront per heavy grid point and ZB slices: lane l sums the entries
+// k0 + l, k0 + l + 64, ... (each in entry order), the 64 partial sums go to
+// LDS, and lane q < ZB adds slice q's partials in lane order -- a fixed
+// summation order, so the result does not depend on scheduling.
+template <int ZB, bool FULL>
+__global__ __launch_bounds__(256) void tomo_grid_gather_heavy_k(TomoArgs a) {
+  constexpr int W = 4, PS = ZB + 1;  // waves per workgroup; padded partial row
+  __shared__ float2 part[W * 64 * PS];
+  const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int hi = blockIdx.x * W + wv;
+  if (hi >= a.nheavy) return;  // the whole wave leaves: no barrier below
+  const int64_t n4 = 4LL * a.n * a.n;
+  const int g = a.heavy[hi];
+  const int z0 = blockIdx.y * ZB;
+  float2 acc[ZB];
+#pragma unroll
+  for (int q = 0; q < ZB; ++q) acc[q] = make_float2(0.f, 0.f);
+  const int k0 = a.rp[g], k1 = a.rp[g + 1];
+  for (int k = k0 + lane; k < k1; k += 64) {
+    const int2 cw = a.csw[k];
+    const float w = __int_as_float(cw.y);
+    const float2* src = a.Sh + (int64_t)cw.x * a.nz + z0;
+    if (FULL) {
+      const float4* s4 = (const float4*)src;
+#pragma unroll
+      for (int q = 0; q < ZB / 2; ++q) {
+        const float4 v = s4[q];
+        acc[2 * q] = vfma(w, make_float2(v.x, v.y), acc[2 * q]);
+        acc[2 * q + 1] = vfma(w, make_float2(v.z, v.w), acc[2 * q + 1]);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < ZB; ++q)
+        if (z0 + q < a.nz) acc[q] = vfma(w, src[q], acc[q]);
+    }
+  }
+  float2* mine = part + (wv * 64 + lane) * PS;
+#pragma unroll
+  for (int q = 0; q < ZB; ++q) mine[q] = acc[q];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane < ZB && (FULL || z0 + lane < a.nz)) {
+    const float2* col = part + wv * 64 * PS + lane;
+    float2 t = make_float2(0.f, 0.f);
+    for (int l = 0; l < 64; ++l) t = cadd(t, col[l * PS]);
+    a.G[(int64_t)(z0 + lane) * n4 + g] = t;
+  }
+}
+
+// The grid gather of the adjoint: thread-per-point rows, then the heavy rows.
+static hipError_t launch_grid_gather(const TomoArgs& a, hipStream_t st) {
+  const int64_t n4 = 4LL * a.n * a.n;
+  const unsigned gx = (unsigned)((n4 + 255) / 256), hx = (unsigned)((a.nheavy + 3) / 4);
+  if (a.nz % 32 == 0) {
+    hipLaunchKernelGGL((tomo_grid_gather_k<32, true>), dim3(gx, (unsigned)(a.nz / 32)), dim3(256), 0, st, a);
+    if (a.nheavy)
+      hipLaunchKernelGGL((tomo_grid_gather_heavy_k<32, true>), dim3(hx, (unsigned)(a.nz / 32)), dim3(256), 0, st, a);
+  } else if (a.nz % 16 == 0) {
+    hipLaunchKernelGGL((tomo_grid_gather_k<16, true>), dim3(gx, (unsigned)(a.nz / 16)), dim3(256), 0, st, a);
+    if (a.nheavy)
+      hipLaunchKernelGGL((tomo_grid_gather_heavy_k<16, true>), dim3(hx, (unsigned)(a.nz / 16)), dim3(256), 0, st, a);
+  } else {
+    const unsigned zb = (unsigned)((a.nz + 15) / 16);
+    hipLaunchKernelGGL((tomo_grid_gather_k<16, false>), dim3(gx, zb), dim3(256), 0, st, a);
+    if (a.nheavy) hipLaunchKernelGGL((tomo_grid_gather_heavy_k<16, false>), dim3(hx, zb), dim3(256), 0, st, a);
+  }
+  return hipGetLastError();
 }
 
 // ---- adjoint pass B': columns of the grid, keep the n central rows ----------
@@ -585,14 +667,7 @@ static hipError_t tomo_adj_any(const TomoArgs& a, const TIO* data, TIO* out, hip
   const int N2 = 2 * a.n;
   hipError_t e = any_launch_rows(TomoLineFftAny<TIO>{a, data}, f1, (int64_t)a.ntheta * a.nz, st);
   if (e != hipSuccess) return e;
-  const unsigned gx = (unsigned)(((int64_t)N2 * N2 + 255) / 256);
-  if (a.nz % 32 == 0)
-    hipLaunchKernelGGL((tomo_grid_gather_k<32, true>), dim3(gx, (unsigned)(a.nz / 32)), dim3(256), 0, st, a);
-  else if (a.nz % 16 == 0)
-    hipLaunchKernelGGL((tomo_grid_gather_k<16, true>), dim3(gx, (unsigned)(a.nz / 16)), dim3(256), 0, st, a);
-  else
-    hipLaunchKernelGGL((tomo_grid_gather_k<16, false>), dim3(gx, (unsigned)((a.nz + 15) / 16)), dim3(256), 0, st, a);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_grid_gather(a, st)) != hipSuccess) return e;
   const int C = any_cols_c(N2, any_max_lds()), groups = (N2 + C - 1) / C;
   if ((e = any_launch_cols(TomoColsAdjAny{a, C, groups}, f2, C, (int64_t)a.nz * groups, st)) != hipSuccess) return e;
   return any_launch_rows(TomoRowsAdjAny<TIO>{a, out}, f2, (int64_t)a.nz * a.n, st);
@@ -650,15 +725,7 @@ static hipError_t tomo_adj_n(const TomoArgs& a, const TIO* data, TIO* out, hipSt
                      data);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const int64_t n4 = (int64_t)N2 * N2;
-  const unsigned gx = (unsigned)((n4 + 255) / 256);
-  if (a.nz % 32 == 0)
-    hipLaunchKernelGGL((tomo_grid_gather_k<32, true>), dim3(gx, (unsigned)(a.nz / 32)), dim3(256), 0, st, a);
-  else if (a.nz % 16 == 0)
-    hipLaunchKernelGGL((tomo_grid_gather_k<16, true>), dim3(gx, (unsigned)(a.nz / 16)), dim3(256), 0, st, a);
-  else
-    hipLaunchKernelGGL((tomo_grid_gather_k<16, false>), dim3(gx, (unsigned)((a.nz + 15) / 16)), dim3(256), 0, st, a);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_grid_gather(a, st)) != hipSuccess) return e;
   hipLaunchKernelGGL((tomo_cols_adj_k<N2, E, C>), dim3((unsigned)(a.nz * (N2 / C))), dim3(C * (N2 / E)), 0, st, a);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const int64_t rows = (int64_t)a.nz * N;
@@ -733,7 +800,8 @@ struct htamd_tomo_plan {
   int4* items = nullptr;
   int2* so = nullptr;
   float2* sf = nullptr;
-  int nitems = 0, ts = 0;
+  int* heavy = nullptr;
+  int nitems = 0, ts = 0, nheavy = 0;
   // detector filters (ramp, Shepp-Logan, Parzen): [3][n] with the 1/n of the
   // inverse transform for htamd_tomo_filter, and [3][n] shifted by n/2 and
   // scaled by sqrt(n / ntheta) for htamd_tomo_fbp
@@ -830,6 +898,13 @@ int htamd_tomo_plan_create(int64_t n, int64_t ntheta, const double* theta, const
       for (int i0 = 0; i0 < L; ++i0)
         ++rp[(size_t)(wrap(tb[(size_t)si].y + i1) * n2 + wrap(tb[(size_t)si].x + i0) + 1)];
   for (int64_t g = 0; g < ng; ++g) rp[(size_t)(g + 1)] += rp[(size_t)g];
+  // grid points with more than kHeavyRow taps, longest first (ties by index)
+  std::vector<int> heavy;
+  for (int64_t g = 0; g < ng; ++g)
+    if (rp[(size_t)(g + 1)] - rp[(size_t)g] > kHeavyRow) heavy.push_back((int)g);
+  std::stable_sort(heavy.begin(), heavy.end(), [&](int x, int y) {
+    return rp[(size_t)x + 1] - rp[(size_t)x] > rp[(size_t)y + 1] - rp[(size_t)y];
+  });
   std::vector<int> fill(rp.begin(), rp.end() - 1);
   std::vector<int2> csw((size_t)(S * L * L));
   for (int64_t si = 0; si < S; ++si)
@@ -872,6 +947,7 @@ int htamd_tomo_plan_create(int64_t n, int64_t ntheta, const double* theta, const
     for (int b = tcount[(size_t)tile]; b < tcount[(size_t)(tile + 1)]; b += 256)
       items.push_back(make_int4(tile % ntx, tile / ntx, b, std::min(256, tcount[(size_t)(tile + 1)] - b)));
   p->nitems = (int)items.size();
+  p->nheavy = (int)heavy.size();
   p->ts = ts;
   // deapodiser phi (tomo.py:19-21, 46-47) with the mask folded in
   std::vector<float> phi((size_t)(n * n));
@@ -909,6 +985,7 @@ int htamd_tomo_plan_create(int64_t n, int64_t ntheta, const double* theta, const
       (e = up(&p->tw1, twid(n))) != hipSuccess || (e = up(&p->tb, tb)) != hipSuccess ||
       (e = up(&p->tw, tw)) != hipSuccess || (e = up(&p->rp, rp)) != hipSuccess || (e = up(&p->csw, csw)) != hipSuccess ||
       (e = up(&p->items, items)) != hipSuccess ||
+      (!heavy.empty() && (e = up(&p->heavy, heavy)) != hipSuccess) ||
       (e = up(&p->so, so)) != hipSuccess || (e = up(&p->sf, sf)) != hipSuccess ||
       (e = up(&p->filt, filt)) != hipSuccess || (e = up(&p->filt_fbp, filt_fbp)) != hipSuccess) {
     htamd_tomo_plan_destroy(p);
@@ -930,6 +1007,7 @@ int htamd_tomo_plan_destroy(htamd_tomo_plan* p) {
   (void)hipFree(p->items);
   (void)hipFree(p->so);
   (void)hipFree(p->sf);
+  (void)hipFree(p->heavy);
   (void)hipFree(p->filt);
   (void)hipFree(p->filt_fbp);
   delete p;
@@ -972,6 +1050,8 @@ static int tomo_run(const htamd_tomo_plan* p, bool adjoint, const void* in, int 
   a.sf = p->sf;
   a.ts = p->ts;
   a.nitems = p->nitems;
+  a.heavy = p->heavy;
+  a.nheavy = p->nheavy;
   a.cxs = (float)(-M_PI * M_PI / p->mu / (4.0 * (double)p->n * (double)p->n));
   a.cys = (float)(M_PI / p->mu);
   a.W = (float2*)ws;

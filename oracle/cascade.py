@@ -40,8 +40,13 @@ class Cascade:
     """Per-chunk cascade of rec_mpi.Rec with the oracle operators."""
 
     def __init__(self, prop, shift, ndist, nz, n, nzobj, nobj, data_size, mag, rho=(1.0, 1.0, 1.0),
-                 obj_real=False):
+                 obj_real=False, single=False):
+        """single: complex64 intermediates (with a complex64 propagator and a
+        single-precision shift), for timing the reference's precision on the
+        CPU; the default is FP64 throughout."""
         self.prop, self.shift = prop, shift
+        self.cdtype = np.complex64 if single else np.complex128
+        self.rdtype = np.float32 if single else np.float64
         self.ndist, self.nz, self.n, self.nzobj, self.nobj = ndist, nz, n, nzobj, nobj
         self.data_size = data_size
         self.mag = np.asarray(mag, dtype=np.float64)  # [B, ndist]
@@ -76,7 +81,7 @@ class Cascade:
     # ---- x0 = F1(x11, x12) = D(x11 x12)   (rec_mpi.py:572-643)
     def F1(self, x):
         x11, x12 = x
-        x0 = np.empty([x12.shape[0], self.ndist, self.nz, self.n], dtype=np.complex128)
+        x0 = np.empty([x12.shape[0], self.ndist, self.nz, self.n], dtype=self.cdtype)
         for j in range(self.ndist):
             x0[:, j] = self.prop.D(x11[j] * x12[:, j], j)
         return x0
@@ -116,7 +121,7 @@ class Cascade:
         for i in range(2, 4)[::-1]:
             x = self.F[i](x)
         x11, x12 = x
-        y12 = np.empty([y0.shape[0], self.ndist, self.nz, self.n], dtype=np.complex128)
+        y12 = np.empty([y0.shape[0], self.ndist, self.nz, self.n], dtype=self.cdtype)
         for j in range(self.ndist):
             y12[:, j] = self.prop.DT(y0[:, j], j)
         y11 = np.sum(y12 * np.conj(x12), axis=0)
@@ -158,7 +163,7 @@ class Cascade:
     def F3(self, x):
         x31, x32, x33 = x
         x22 = np.empty([len(x33), self.ndist, self.nz, self.n],
-                       dtype=np.float64 if self.obj_real else np.complex128)
+                       dtype=self.rdtype if self.obj_real else self.cdtype)
         c = self.shift.coeff(x32)
         for k in range(self.ndist):
             x22[:, k] = self.shift.curlySc(c, x33[:, k], self.mag[:, k])
@@ -196,7 +201,7 @@ class Cascade:
         y21, y22 = y
         x31, x32, x33 = x
         y32 = np.zeros([y22.shape[0], self.nzobj, self.nobj],
-                       dtype=np.float64 if self.obj_real else np.complex128)
+                       dtype=self.rdtype if self.obj_real else self.cdtype)
         y33 = np.empty([y22.shape[0], self.ndist, 2])
         c = self.shift.coeff(x32)
         for k in range(self.ndist):

@@ -72,12 +72,15 @@ def ifB3(nzpsi, npsi):
     return 1 / np.fft.fftshift(np.outer(divy, divx), axes=(-1, -2))
 
 
-def coeff(psi, real=False):
-    """B-spline coefficients: ifft2(fft2(psi) * ifB3)  (shift.py:74-82)."""
+def coeff(psi, real=False, single=False):
+    """B-spline coefficients: ifft2(fft2(psi) * ifB3)  (shift.py:74-82).
+    single: complex64 transforms (the reference's precision), else complex128."""
     psi = np.asarray(psi)
     nzpsi, npsi = psi.shape[-2:]
-    out = sfft.ifft2(sfft.fft2(psi, axes=(-2, -1), workers=WORKERS) * ifB3(nzpsi, npsi),
-                     axes=(-2, -1), workers=WORKERS)
+    h = ifB3(nzpsi, npsi)
+    if single:
+        psi, h = psi.astype(np.complex64), h.astype(np.float32)
+    out = sfft.ifft2(sfft.fft2(psi, axes=(-2, -1), workers=WORKERS) * h, axes=(-2, -1), workers=WORKERS)
     return out.real if real else out
 
 
@@ -110,21 +113,28 @@ def coords(n, nz, npsi, nzpsi, r, m):
 
 
 class Shift:
-    """Restatement of Shift (shift.py:15-221) on float64 / complex128."""
+    """Restatement of Shift (shift.py:15-221) on float64 / complex128, or with
+    single=True on float32 / complex64 (FP32 weights and sums, for timing the
+    reference's precision on the CPU; the sampling coordinates stay FP64)."""
 
-    def __init__(self, n, npsi, nz, nzpsi, obj_dtype="complex64", nchunk=None):
+    def __init__(self, n, npsi, nz, nzpsi, obj_dtype="complex64", nchunk=None, single=False):
         self.n, self.npsi, self.nz, self.nzpsi = n, npsi, nz, nzpsi
         self.real = obj_dtype == "float32"
-        self.dtype = np.float64 if self.real else np.complex128
+        self.single = single
+        if single:
+            self.dtype = np.float32 if self.real else np.complex64
+        else:
+            self.dtype = np.float64 if self.real else np.complex128
 
     def coeff(self, psi):
-        return coeff(psi, real=self.real)
+        return coeff(psi, real=self.real, single=self.single)
 
     def _mats(self, r, m, kinds=(0,)):
         x, y = coords(self.n, self.nz, self.npsi, self.nzpsi, r, m)
         B = x.shape[0]
-        wx = [[axis_matrix(self.n, self.npsi, x[b], k) for k in kinds] for b in range(B)]
-        wy = [[axis_matrix(self.nz, self.nzpsi, y[b], k) for k in kinds] for b in range(B)]
+        wt = np.float32 if self.single else np.float64
+        wx = [[axis_matrix(self.n, self.npsi, x[b], k).astype(wt) for k in kinds] for b in range(B)]
+        wy = [[axis_matrix(self.nz, self.nzpsi, y[b], k).astype(wt) for k in kinds] for b in range(B)]
         return wx, wy
 
     @staticmethod

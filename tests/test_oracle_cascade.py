@@ -95,3 +95,31 @@ def test_taylor_ratios_F3():
     assert e1[0] / e1[1] == pytest.approx(4, rel=0.15)
     assert e1[1] / e1[2] == pytest.approx(4, rel=0.15)
     assert e2[0] / e2[1] == pytest.approx(8, rel=0.2)
+
+
+def test_single_precision_mode_tracks_fp64():
+    """The oracle's single-precision mode (used only to time the reference's
+    precision on the CPU) computes in complex64 and stays within FP32 rounding
+    of the FP64 oracle: gradient relative L2 <= 1e-4."""
+    from oracle import cascade as oc
+    from oracle import propagation as op
+    from oracle import shift as osh
+    n, nobj, nd, B = 32, 24, 2, 2
+    rng = np.random.default_rng(77)
+    mag = rng.uniform(1.0, 1.1, (B, nd))
+    dist = [40.0, 55.0]
+    out = []
+    for single in (False, True):
+        cas = oc.Cascade(op.Propagation(n, n, B, nd, 1.0, 1.0, dist, dtype=np.complex64 if single else np.complex128),
+                         osh.Shift(n, nobj, n, nobj, single=single), nd, n, n, nobj, nobj, B * nd * n * n, mag,
+                         rho=(1.0, 0.5, 0.2), single=single)
+        rng = np.random.default_rng(78)
+        prb = np.exp(0.2j * rng.standard_normal((nd, n, n)))
+        proj = 0.3 * (rng.standard_normal((B, nobj, nobj)) + 1j * rng.standard_normal((B, nobj, nobj)))
+        pos = rng.uniform(-2, 2, (B, nd, 2))
+        d = rng.uniform(0.5, 1.5, (B, nd, n, n))
+        cast = (lambda a: a.astype(np.complex64)) if single else (lambda a: a)
+        out.append(cas.gradients(cast(prb), cast(proj), pos, d.astype(np.float32) if single else d))
+    for g64, g32 in zip(*out):
+        assert np.linalg.norm(g32 - g64) / np.linalg.norm(g64) < 1e-4
+    assert out[1][0].dtype == np.complex64 and out[1][1].dtype == np.complex64
