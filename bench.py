@@ -841,6 +841,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     fn_plain = lambda: plain_step(rec, vars_, grads, data)  # noqa: E731
+    fused_value = None
     if path == "plain":
         wall, per = time_steps(fn_plain, a.steps, a.warmup)
         plain_value = value = a.angles * a.steps / wall
@@ -850,10 +851,14 @@ def main():
             n_plain = max(2, a.steps // 2)
             wall_p, _ = time_steps(fn_plain, n_plain, 1)
             plain_value = a.angles * n_plain / wall_p
+        fn = lambda: fused_step(rec, vars_, grads, data)  # noqa: E731
+        fused_value = None
         if path == "fused":
-            fn = lambda: fused_step(rec, vars_, grads, data)  # noqa: E731
             wall, per = time_steps(fn, a.steps, a.warmup)
         else:
+            # the same fused launches called back to back, for comparison
+            wall_f, _ = time_steps(fn, a.steps, a.warmup)
+            fused_value = a.angles * a.steps / wall_f
             wall, per = served_run(a, rec, vars_, data, a.steps, a.warmup)
         value = a.angles * a.steps / wall
     ms = wall * 1e3 / a.steps
@@ -872,6 +877,9 @@ def main():
         "speedup_vs_plain": round(value / plain_value, 3) if plain_value else None,
         "roofline": roof,
     }
+    if path == "served" and fused_value:
+        out["fused_value"] = round(fused_value, 3)
+        out["served_vs_fused"] = round(value / fused_value, 3)
     if isinstance(roof, dict):
         roof["measured_copy_gbs"] = measured_copy_gbs()  # beside the 8 TB/s spec peak
     if not a.no_cpu_baseline:

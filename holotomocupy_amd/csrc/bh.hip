@@ -28,6 +28,7 @@ struct BhArgs {
   const float2* prb = nullptr;       // [nprb, ndist, nz, n]
   const int* pidx = nullptr;         // [nang] probe of each angle (NULL: probe 0)
   const float* data = nullptr;       // [nang, ndist, nz, n] sqrt(intensity)
+  const float* const* data_p = nullptr;  // or per angle: data_p[b] -> [ndist, nz, n]
   const float4* scales = nullptr;    // [nang]: (2/data_size, rho_obj^2, rho_pos^2, unused)
   float2* T = nullptr;               // work field [nang, ndist, nz, n]
   float2* gprb = nullptr;            // [nprb, ndist, nz, n]
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_f
   FftTw<N, E, RM> twr;
   fft_load_twiddles<N, E, RM>(twr, t, a.twy);
   float2* base = a.T + f * (int64_t)N * a.n + x0;
-  const float* dbase = a.data + f * (int64_t)N * a.n + x0;
+  const float* dbase = (a.data_p ? a.data_p[b] + (int64_t)j * N * a.n : a.data + f * (int64_t)N * a.n) + x0;
   col_tile_xfer<N, E, C, true>(lds, base, a.n, threadIdx.x);  // FFT slot q holds row dct_nat(q)
   __syncthreads();
   float2* s = lds + cc * LS;
@@ -772,15 +773,15 @@ int htamd_bh_workspace_size(const htamd_bh_plan* p, int64_t nangles, int64_t* by
   return HTAMD_OK;
 }
 
-int htamd_bh_gradient(const htamd_bh_plan* p, int64_t nangles, const void* proj, const float* pos,
-                      const float* mag, const void* prb, const int* probe_index, const float* data,
-                      const float* angle_scales, const float* rho_prb, int accumulate_prb, void* gproj,
-                      float* gpos, void* gprb, double* objective, void* workspace,
-                      int64_t workspace_bytes, void* stream) {
+static int bh_gradient_impl(const htamd_bh_plan* p, int64_t nangles, const void* proj, const float* pos,
+                            const float* mag, const void* prb, const int* probe_index, const float* data,
+                            const float* const* data_ptrs, const float* angle_scales, const float* rho_prb,
+                            int accumulate_prb, void* gproj, float* gpos, void* gprb, double* objective,
+                            void* workspace, int64_t workspace_bytes, void* stream) {
   if (!p) return fail(HTAMD_ERR_ARG, "bh_gradient: plan is NULL");
   if (nangles < 0) return fail(HTAMD_ERR_ARG, "bh_gradient: negative angle count");
   if (nangles == 0) return HTAMD_OK;
-  if (!proj || !pos || !mag || !prb || !data || !angle_scales || !rho_prb || !gproj || !gpos || !gprb)
+  if (!proj || !pos || !mag || !prb || !(data || data_ptrs) || !angle_scales || !rho_prb || !gproj || !gpos || !gprb)
     return fail(HTAMD_ERR_ARG, "bh_gradient: NULL buffer");
   const BhLayout L = bh_layout(p->prop, p->shift, nangles);
   if (!workspace || workspace_bytes < L.total)
@@ -802,6 +803,7 @@ int htamd_bh_gradient(const htamd_bh_plan* p, int64_t nangles, const void* proj,
   a.prb = (const float2*)prb;
   a.pidx = probe_index;
   a.data = data;
+  a.data_p = data_ptrs;
   a.scales = (const float4*)angle_scales;
   a.T = (float2*)(ws + L.T);
   a.gprb = (float2*)gprb;
@@ -817,6 +819,25 @@ int htamd_bh_gradient(const htamd_bh_plan* p, int64_t nangles, const void* proj,
       ? bh_run<float2>(p->shift, a, proj, gproj, gpos, objective, L, ws, (hipStream_t)stream, p->events)
       : bh_run<float>(p->shift, a, proj, gproj, gpos, objective, L, ws, (hipStream_t)stream, p->events);
   return e == hipSuccess ? HTAMD_OK : hip_fail(e, "bh_gradient");
+}
+
+int htamd_bh_gradient(const htamd_bh_plan* p, int64_t nangles, const void* proj, const float* pos,
+                      const float* mag, const void* prb, const int* probe_index, const float* data,
+                      const float* angle_scales, const float* rho_prb, int accumulate_prb, void* gproj,
+                      float* gpos, void* gprb, double* objective, void* workspace,
+                      int64_t workspace_bytes, void* stream) {
+  return bh_gradient_impl(p, nangles, proj, pos, mag, prb, probe_index, data, nullptr, angle_scales, rho_prb,
+                          accumulate_prb, gproj, gpos, gprb, objective, workspace, workspace_bytes, stream);
+}
+
+int htamd_bh_gradient_dp(const htamd_bh_plan* p, int64_t nangles, const void* proj, const float* pos,
+                         const float* mag, const void* prb, const int* probe_index, const float* const* data_ptrs,
+                         const float* angle_scales, const float* rho_prb, int accumulate_prb, void* gproj,
+                         float* gpos, void* gprb, double* objective, void* workspace, int64_t workspace_bytes,
+                         void* stream) {
+  if (!data_ptrs) return fail(HTAMD_ERR_ARG, "bh_gradient_dp: NULL data pointer table");
+  return bh_gradient_impl(p, nangles, proj, pos, mag, prb, probe_index, nullptr, data_ptrs, angle_scales, rho_prb,
+                          accumulate_prb, gproj, gpos, gprb, objective, workspace, workspace_bytes, stream);
 }
 
 }  // extern "C"

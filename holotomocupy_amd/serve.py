@@ -366,14 +366,14 @@ class GpuExecutor:
         f32, nd = torch.float32, geo.ndist
         return [("proj", (A, geo.nzobj, geo.nobj), cdt), ("gproj", (A, geo.nzobj, geo.nobj), cdt),
                 ("pos", (A, nd, 2), f32), ("gpos", (A, nd, 2), f32), ("mag", (A, nd), f32),
-                ("data", (A, nd, geo.nz, geo.n), f32), ("scales", (A, 4), f32), ("pidx", (A,), torch.int32),
+                ("dptr", (A,), torch.int64), ("scales", (A, 4), f32), ("pidx", (A,), torch.int32),
                 ("prb", (P, nd, geo.nz, geo.n), torch.complex64), ("gprb", (P, nd, geo.nz, geo.n), torch.complex64),
                 ("rho_prb", (P,), f32)]
 
     def _gradient_arena(self, si, b, geo, A, P):
         torch = self.torch
         parts = self._staging_parts(geo, A, P)
-        esz = {torch.complex64: 8, torch.float32: 4, torch.int32: 4}
+        esz = {torch.complex64: 8, torch.float32: 4, torch.int32: 4, torch.int64: 8}
         span = self._span(parts, esz)
         ws = b["bh"].workspace_bytes(A)
         arena = self._arena(si, span + ws)
@@ -382,9 +382,9 @@ class GpuExecutor:
         return st
 
     def _launch(self, b, st, A, P):
-        b["bh"](st["prb"], st["proj"], st["pos"], st["data"], st["mag"], st["gproj"], st["gpos"], st["gprb"],
+        b["bh"](st["prb"], st["proj"], st["pos"], None, st["mag"], st["gproj"], st["gpos"], st["gprb"],
                 accumulate_prb=False, probe_index=st["pidx"], scales=st["scales"], rho_prb=st["rho_prb"],
-                workspace=st["ws"])
+                workspace=st["ws"], data_ptrs=st["dptr"])
 
     # ---- batches
     def _run_prop(self, key, batch, s):
@@ -481,13 +481,20 @@ class GpuExecutor:
             st = self._gradient_arena(si, b, geo, A, P)
             mag = b["rec"].eff_demagnifications[0]
             o = 0
+            # the fused passes read each request's data where it lies, through
+            # a table of per-angle addresses (the data are 90% of a gradient
+            # request's bytes; the other inputs are packed)
+            ptrs = []
             for p, r in enumerate(batch):
                 a = r.nangles
+                if not r.data.is_contiguous():
+                    r.data = r.data.contiguous()
                 for t in (r.prb, r.proj, r.pos, r.data):
                     t.record_stream(s)
+                step = r.data[0].numel() * r.data.element_size()
+                ptrs.extend(r.data.data_ptr() + i * step for i in range(a))
                 st["proj"][o:o + a].copy_(r.proj, non_blocking=True)
                 st["pos"][o:o + a].copy_(r.pos, non_blocking=True)
-                st["data"][o:o + a].copy_(r.data, non_blocking=True)
                 st["mag"][o:o + a].copy_(mag.expand(a, -1), non_blocking=True)
                 for c, v in enumerate((2.0 / r.data_size, r.rho[0] ** 2, r.rho[2] ** 2, 0.0)):
                     st["scales"][o:o + a, c].fill_(v)  # device fills: no host copies on the stream
@@ -495,6 +502,8 @@ class GpuExecutor:
                 st["prb"][p].copy_(r.prb, non_blocking=True)
                 st["rho_prb"][p].fill_(r.rho[1] ** 2)
                 o += a
+            # from pinned memory, which the host allocator keeps until the copy has run
+            st["dptr"].copy_(torch.tensor(ptrs, dtype=torch.int64).pin_memory(), non_blocking=True)
             key = (geo, A, P)
             g = slot["graphs"].get(key) if self.use_graphs else None
             if g is not None:

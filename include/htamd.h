@@ -242,6 +242,16 @@ int htamd_bh_gradient(const htamd_bh_plan* plan, int64_t nangles, const void* pr
                       float* gpos, void* gprb, double* objective, void* workspace,
                       int64_t workspace_bytes, void* stream);
 
+/* As htamd_bh_gradient with the data of angle b read at data_ptrs[b] ([ndist, nz, n] float32 each):
+ * data_ptrs is a device array of nangles device pointers, so a batch of requests is served from the
+ * requests' own data buffers instead of a packed copy.  No reference counterpart (the reference
+ * evaluates one chunk of one array, rec_mpi.py:327-353); it exists for the serving loop. */
+int htamd_bh_gradient_dp(const htamd_bh_plan* plan, int64_t nangles, const void* proj, const float* pos,
+                         const float* mag, const void* prb, const int* probe_index, const float* const* data_ptrs,
+                         const float* angle_scales, const float* rho_prb, int accumulate_prb, void* gproj,
+                         float* gpos, void* gprb, double* objective, void* workspace, int64_t workspace_bytes,
+                         void* stream);
+
 /* Fused BH bilinear form of the data term.  Replaces Rec.hessian_cascade / _hessian_cascade and the
  * dF1..dF3, d2F_dF0..d2F_dF3 chain (src/holotomocupy/rec_mpi.py:250-303, 517-813) for one chunk of
  * angles: result (+)= scale * sum [(1 - d/|x0|) Re(conj(y0) z0) + d/|x0|^3 Re(conj(x0) y0) Re(conj(x0) z0)
