@@ -36,7 +36,7 @@ SIGNATURES = {
     "htamd_bh_workspace_size": [c_vp, c_i64, ctypes.POINTER(c_i64)],
     "htamd_bh_set_stage_events": [c_vp, ctypes.POINTER(c_vp), c_int],
     "htamd_bh_gradient": [c_vp, c_i64] + [c_vp] * 8 + [c_int] + [c_vp] * 5 + [c_i64, c_vp],
-    "htamd_bh_gradient_dp": [c_vp, c_i64] + [c_vp] * 8 + [c_int] + [c_vp] * 5 + [c_i64, c_vp],
+    "htamd_bh_gradient_tab": [c_vp, c_i64, c_i64] + [c_vp] * 6 + [c_int, c_vp, c_vp, c_vp, c_i64, c_vp],
     "htamd_bh_hessian_workspace_size": [c_vp, c_i64, c_int, ctypes.POINTER(c_i64)],
     "htamd_bh_hessian": [c_vp, c_i64] + [c_vp] * 11 + [c_dbl, c_vp, c_int, c_vp, c_i64, c_vp],
     "htamd_bh_hessian_pair": [c_vp, c_i64] + [c_vp] * 11 + [c_dbl, c_vp, c_int, c_vp, c_i64, c_vp],

@@ -51,34 +51,39 @@ class BHGradient:
         return self._scales[key]
 
     def __call__(self, prb, proj, pos, data, mag, gproj, gpos, gprb, accumulate_prb=True,
-                 objective=None, probe_index=None, scales=None, rho_prb=None, workspace=None, data_ptrs=None):
+                 objective=None, probe_index=None, scales=None, rho_prb=None, workspace=None):
         """Write gproj / gpos and accumulate (or overwrite) gprb for the angles
         of proj; all arguments are device tensors with the layouts of
-        include/htamd.h.  objective: optional float64 device tensor [1].
-        data_ptrs (with data None): int64 device tensor [B] of per-angle data
-        addresses (htamd_bh_gradient_dp)."""
+        include/htamd.h.  objective: optional float64 device tensor [1]."""
         B = proj.shape[0]
-        if (data is None) == (data_ptrs is None):
-            raise ValueError("pass either data or data_ptrs")
-        src = ("data", data) if data is not None else ("data_ptrs", data_ptrs)
-        for t, name in ((prb, "prb"), (proj, "proj"), (pos, "pos"), (src[1], src[0]), (mag, "mag"),
+        for t, name in ((prb, "prb"), (proj, "proj"), (pos, "pos"), (data, "data"), (mag, "mag"),
                         (gproj, "gproj"), (gpos, "gpos"), (gprb, "gprb")):
             if not t.is_contiguous() or t.device != self.device:
                 raise ValueError(f"{name} must be a contiguous tensor on {self.device}")
-        if data_ptrs is not None and (data_ptrs.dtype != torch.int64 or data_ptrs.numel() != B):
-            raise ValueError(f"data_ptrs must be an int64 tensor of {B} addresses")
         if scales is None or rho_prb is None:
             s, rp = self.scales(B)
             scales = s if scales is None else scales
             rho_prb = rp if rho_prb is None else rho_prb
         ws = self.workspace(B) if workspace is None else workspace
-        _lib.call("htamd_bh_gradient" if data is not None else "htamd_bh_gradient_dp", self._plan, B,
-                  _lib.ptr(proj), _lib.ptr(pos), _lib.ptr(mag),
+        _lib.call("htamd_bh_gradient", self._plan, B, _lib.ptr(proj), _lib.ptr(pos), _lib.ptr(mag),
                   _lib.ptr(prb), _lib.ptr(probe_index) if probe_index is not None else None,
-                  _lib.ptr(src[1]), _lib.ptr(scales), _lib.ptr(rho_prb), 1 if accumulate_prb else 0,
+                  _lib.ptr(data), _lib.ptr(scales), _lib.ptr(rho_prb), 1 if accumulate_prb else 0,
                   _lib.ptr(gproj), _lib.ptr(gpos), _lib.ptr(gprb),
                   _lib.ptr(objective) if objective is not None else None,
                   _lib.ptr(ws), ws.numel(), _lib.stream_ptr())
+
+    def call_tab(self, tables, nangles, nprobes, pos, mag, probe_index, scales, rho_prb, gpos, workspace,
+                 accumulate_prb=False, objective=None):
+        """htamd_bh_gradient_tab: the projections, data, object gradients,
+        probes and probe gradients of the batch addressed through `tables`
+        (int64 device tensor of 3 nangles + 2 nprobes addresses); pos, mag,
+        probe_index, scales, rho_prb and gpos packed as for __call__."""
+        if tables.dtype != torch.int64 or tables.numel() != 3 * nangles + 2 * nprobes:
+            raise ValueError(f"tables must hold {3 * nangles + 2 * nprobes} int64 addresses")
+        _lib.call("htamd_bh_gradient_tab", self._plan, int(nangles), int(nprobes), _lib.ptr(tables), _lib.ptr(pos),
+                  _lib.ptr(mag), _lib.ptr(probe_index), _lib.ptr(scales), _lib.ptr(rho_prb),
+                  1 if accumulate_prb else 0, _lib.ptr(gpos), _lib.ptr(objective) if objective is not None else None,
+                  _lib.ptr(workspace), workspace.numel(), _lib.stream_ptr())
 
     def hessian_workspace_bytes(self, nangles, same):
         """same: False (general form), True (z = y) or 2 (the pair form)."""

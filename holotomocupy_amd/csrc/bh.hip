@@ -29,6 +29,8 @@ struct BhArgs {
   const int* pidx = nullptr;         // [nang] probe of each angle (NULL: probe 0)
   const float* data = nullptr;       // [nang, ndist, nz, n] sqrt(intensity)
   const float* const* data_p = nullptr;  // or per angle: data_p[b] -> [ndist, nz, n]
+  const float2* const* prb_p = nullptr;  // or per probe slot: prb_p[p] -> [ndist, nz, n]
+  float2* const* gprb_p = nullptr;       // or per probe slot: gprb_p[p] -> [ndist, nz, n]
   const float4* scales = nullptr;    // [nang]: (2/data_size, rho_obj^2, rho_pos^2, unused)
   float2* T = nullptr;               // work field [nang, ndist, nz, n]
   float2* gprb = nullptr;            // [nprb, ndist, nz, n]
@@ -180,7 +182,7 @@ __global__ __launch_bounds__(kRowThreads) void bh_fwd_rows_k(BhArgs a) {
     bs_w(py.d, wy);
     const TO* cb = (const TO*)a.c + b * plane;
     const int p = a.pidx ? a.pidx[b] : 0;
-    const float2* prow = a.prb + (((int64_t)p * a.ndist + j) * a.nz + ty) * N;
+    const float2* prow = (a.prb_p ? a.prb_p[p] : a.prb + (int64_t)p * a.ndist * a.nz * N) + ((int64_t)j * a.nz + ty) * N;
     if constexpr (STAGED) {
       TO* cy = (TO*)dyn_lds() + r * a.npsi;
       int lo, hi;
@@ -313,7 +315,7 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
   int pcur = a.pidx ? a.pidx[0] : 0;
   auto flush = [&](int p) {
     if (!valid) return;
-    float2* g = a.gprb + (((int64_t)p * a.ndist + j) * a.nz + ty) * N;
+    float2* g = (a.gprb_p ? a.gprb_p[p] : a.gprb + (int64_t)p * a.ndist * a.nz * N) + ((int64_t)j * a.nz + ty) * N;
     const int t = t0;
     const float rho = a.rho_prb[p];
     for (int e = 0; e < E; ++e) {
@@ -355,7 +357,7 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
       flush(pcur);
       pcur = p;
     }
-    const float2* prow = a.prb + (((int64_t)p * a.ndist + j) * a.nz + ty) * N;
+    const float2* prow = (a.prb_p ? a.prb_p[p] : a.prb + (int64_t)p * a.ndist * a.nz * N) + ((int64_t)j * a.nz + ty) * N;
     float s1 = 0.f, s2 = 0.f;
 #pragma unroll 1
     for (int e = 0; e < E; ++e) {
@@ -687,15 +689,20 @@ static hipError_t cols_dispatch(const BhArgs& a, hipStream_t st) {
   }
 }
 
+// proj_p / gproj_p: optional per-angle addresses of the projections and of
+// their gradients (NULL: proj / gproj are [nang, nzpsi, npsi] arrays)
 template <typename TO>
 static hipError_t bh_run(const htamd_shift_plan* sp, BhArgs a, const void* proj, void* gproj,
                          float* gpos, double* objective, const BhLayout& L, char* ws, hipStream_t st,
-                         const StageEvents& sev) {
+                         const StageEvents& sev, const void* const* proj_p = nullptr,
+                         void* const* gproj_p = nullptr) {
   hipError_t e;
   // coefficients of the projections (coeff, shift.py:74-82)
   sev.mark(0, st);
+  PlaneTab<TO> tin;
+  tin.in = (const TO* const*)proj_p;
   if ((e = bs_launch_prefilter<TO>((const TO*)proj, (TO*)(ws + L.c), a.nang, a.nzpsi, a.npsi, sp->taps,
-                                   st)) != hipSuccess)
+                                   st, tin)) != hipSuccess)
     return e;
   sev.mark(1, st);
   sev.mark(2, st);
@@ -714,7 +721,9 @@ static hipError_t bh_run(const htamd_shift_plan* sp, BhArgs a, const void* proj,
   if ((e = hipGetLastError()) != hipSuccess) return e;
   sev.mark(9, st);
   sev.mark(10, st);
-  if ((e = bs_launch_prefilter<TO>(dpsi, (TO*)gproj, a.nang, a.nzpsi, a.npsi, sp->taps, st)) != hipSuccess)
+  PlaneTab<TO> tout;
+  tout.out = (TO* const*)gproj_p;
+  if ((e = bs_launch_prefilter<TO>(dpsi, (TO*)gproj, a.nang, a.nzpsi, a.npsi, sp->taps, st, tout)) != hipSuccess)
     return e;
   sev.mark(11, st);
   sev.mark(12, st);
@@ -773,16 +782,22 @@ int htamd_bh_workspace_size(const htamd_bh_plan* p, int64_t nangles, int64_t* by
   return HTAMD_OK;
 }
 
+// Either the packed arrays (proj, prb, data, gproj, gprb) or, with tab != NULL,
+// the address table tab = [proj x nangles | data x nangles | gproj x nangles |
+// prb x nprobes | gprb x nprobes] (device memory).
 static int bh_gradient_impl(const htamd_bh_plan* p, int64_t nangles, const void* proj, const float* pos,
                             const float* mag, const void* prb, const int* probe_index, const float* data,
-                            const float* const* data_ptrs, const float* angle_scales, const float* rho_prb,
-                            int accumulate_prb, void* gproj, float* gpos, void* gprb, double* objective,
-                            void* workspace, int64_t workspace_bytes, void* stream) {
+                            const void* const* tab, int64_t nprobes, const float* angle_scales,
+                            const float* rho_prb, int accumulate_prb, void* gproj, float* gpos, void* gprb,
+                            double* objective, void* workspace, int64_t workspace_bytes, void* stream) {
   if (!p) return fail(HTAMD_ERR_ARG, "bh_gradient: plan is NULL");
   if (nangles < 0) return fail(HTAMD_ERR_ARG, "bh_gradient: negative angle count");
   if (nangles == 0) return HTAMD_OK;
-  if (!proj || !pos || !mag || !prb || !(data || data_ptrs) || !angle_scales || !rho_prb || !gproj || !gpos || !gprb)
+  if (tab ? (nprobes < 1 || !pos || !mag || !angle_scales || !rho_prb || !gpos)
+          : (!proj || !pos || !mag || !prb || !data || !angle_scales || !rho_prb || !gproj || !gpos || !gprb))
     return fail(HTAMD_ERR_ARG, "bh_gradient: NULL buffer");
+  if (tab && nprobes > 1 && !probe_index)
+    return fail(HTAMD_ERR_ARG, "bh_gradient_tab: several probes need a probe index");
   const BhLayout L = bh_layout(p->prop, p->shift, nangles);
   if (!workspace || workspace_bytes < L.total)
     return fail(HTAMD_ERR_ARG, "bh_gradient: workspace of " + std::to_string(L.total) + " bytes required");
@@ -803,7 +818,15 @@ static int bh_gradient_impl(const htamd_bh_plan* p, int64_t nangles, const void*
   a.prb = (const float2*)prb;
   a.pidx = probe_index;
   a.data = data;
-  a.data_p = data_ptrs;
+  const void* const* proj_p = nullptr;
+  void* const* gproj_p = nullptr;
+  if (tab) {
+    proj_p = tab;
+    a.data_p = (const float* const*)(tab + nangles);
+    gproj_p = (void* const*)(tab + 2 * nangles);
+    a.prb_p = (const float2* const*)(tab + 3 * nangles);
+    a.gprb_p = (float2* const*)(tab + 3 * nangles + nprobes);
+  }
   a.scales = (const float4*)angle_scales;
   a.T = (float2*)(ws + L.T);
   a.gprb = (float2*)gprb;
@@ -816,8 +839,10 @@ static int bh_gradient_impl(const htamd_bh_plan* p, int64_t nangles, const void*
   a.tabx = p->prop->tab_x;
   a.taby = p->prop->tab_y;
   hipError_t e = p->shift->is_complex
-      ? bh_run<float2>(p->shift, a, proj, gproj, gpos, objective, L, ws, (hipStream_t)stream, p->events)
-      : bh_run<float>(p->shift, a, proj, gproj, gpos, objective, L, ws, (hipStream_t)stream, p->events);
+      ? bh_run<float2>(p->shift, a, proj, gproj, gpos, objective, L, ws, (hipStream_t)stream, p->events, proj_p,
+                       gproj_p)
+      : bh_run<float>(p->shift, a, proj, gproj, gpos, objective, L, ws, (hipStream_t)stream, p->events, proj_p,
+                      gproj_p);
   return e == hipSuccess ? HTAMD_OK : hip_fail(e, "bh_gradient");
 }
 
@@ -826,18 +851,18 @@ int htamd_bh_gradient(const htamd_bh_plan* p, int64_t nangles, const void* proj,
                       const float* angle_scales, const float* rho_prb, int accumulate_prb, void* gproj,
                       float* gpos, void* gprb, double* objective, void* workspace,
                       int64_t workspace_bytes, void* stream) {
-  return bh_gradient_impl(p, nangles, proj, pos, mag, prb, probe_index, data, nullptr, angle_scales, rho_prb,
+  return bh_gradient_impl(p, nangles, proj, pos, mag, prb, probe_index, data, nullptr, 0, angle_scales, rho_prb,
                           accumulate_prb, gproj, gpos, gprb, objective, workspace, workspace_bytes, stream);
 }
 
-int htamd_bh_gradient_dp(const htamd_bh_plan* p, int64_t nangles, const void* proj, const float* pos,
-                         const float* mag, const void* prb, const int* probe_index, const float* const* data_ptrs,
-                         const float* angle_scales, const float* rho_prb, int accumulate_prb, void* gproj,
-                         float* gpos, void* gprb, double* objective, void* workspace, int64_t workspace_bytes,
-                         void* stream) {
-  if (!data_ptrs) return fail(HTAMD_ERR_ARG, "bh_gradient_dp: NULL data pointer table");
-  return bh_gradient_impl(p, nangles, proj, pos, mag, prb, probe_index, nullptr, data_ptrs, angle_scales, rho_prb,
-                          accumulate_prb, gproj, gpos, gprb, objective, workspace, workspace_bytes, stream);
+int htamd_bh_gradient_tab(const htamd_bh_plan* p, int64_t nangles, int64_t nprobes, const void* const* tables,
+                          const float* pos, const float* mag, const int* probe_index, const float* angle_scales,
+                          const float* rho_prb, int accumulate_prb, float* gpos, double* objective, void* workspace,
+                          int64_t workspace_bytes, void* stream) {
+  if (!tables) return fail(HTAMD_ERR_ARG, "bh_gradient_tab: NULL address table");
+  return bh_gradient_impl(p, nangles, nullptr, pos, mag, nullptr, probe_index, nullptr, tables, nprobes,
+                          angle_scales, rho_prb, accumulate_prb, nullptr, gpos, nullptr, objective, workspace,
+                          workspace_bytes, stream);
 }
 
 }  // extern "C"

@@ -292,7 +292,7 @@ constexpr int kPfTX = 64, kPfTY = 32, kPfR = 8;
 template <typename T>
 __global__ __launch_bounds__(256) void bs_prefilter_k(const T* __restrict__ in, T* __restrict__ out,
                                                       int nzpsi, int npsi, int tiles_x, int tiles_y,
-                                                      PrefilterTaps taps) {
+                                                      PrefilterTaps taps, PlaneTab<T> tab) {
   constexpr int H = kPfH;
   constexpr int LX = kPfTX + 2 * H, LY = kPfTY + 2 * H;
   __shared__ T s_in[LY][LX + 1];
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(256) void bs_prefilter_k(const T* __restrict__ in, 
   const int64_t b = rest / tiles_y;
   const int x0 = txt * kPfTX, y0 = tyt * kPfTY;
   const int64_t plane = (int64_t)nzpsi * npsi;
-  const T* src = in + b * plane;
+  const T* src = tab.in ? tab.in[b] : in + b * plane;
   for (int i = threadIdx.x; i < LY * LX; i += 256) {
     const int ly = i / LX, lx = i % LX;
     int gy = (y0 + ly - H) % nzpsi;
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(256) void bs_prefilter_k(const T* __restrict__ in, 
     }
   }
   __syncthreads();
-  T* dst = out + b * plane;
+  T* dst = tab.out ? tab.out[b] : out + b * plane;
   for (int i = threadIdx.x; i < kPfTX * (kPfTY / R); i += 256) {
     const int lx = i % kPfTX, gy = (i / kPfTX) * R;
     if (x0 + lx >= npsi) continue;
@@ -367,7 +367,7 @@ static_assert(kPsR * (kPsTX / 8) <= kPsNT, "one x work item per thread");
 template <typename T>
 __global__ __launch_bounds__(kPsNT) void bs_prefilter_strip_k(const T* __restrict__ in, T* __restrict__ out,
                                                               int nzpsi, int npsi, int tiles_x, int strips,
-                                                              int sy, PrefilterTaps taps) {
+                                                              int sy, PrefilterTaps taps, PlaneTab<T> tab) {
   constexpr int H = kPfH, R = kPsR, WIN = kPsR + 2 * kPfH, G = kPsTX / 8;
   __shared__ T tmp[R * kPsRS];
   const int64_t bid = blockIdx.x;
@@ -381,7 +381,7 @@ __global__ __launch_bounds__(kPsNT) void bs_prefilter_strip_k(const T* __restric
   const bool loader = t < kPsW;
   int gx = (x0 - H + t) % npsi;
   gx = gx < 0 ? gx + npsi : gx;
-  const T* src = in + b * plane + gx;
+  const T* src = (tab.in ? tab.in[b] : in + b * plane) + gx;
   int gy = (y0 - H) % nzpsi;
   gy = gy < 0 ? gy + nzpsi : gy;
   T win[WIN];
@@ -390,7 +390,7 @@ __global__ __launch_bounds__(kPsNT) void bs_prefilter_strip_k(const T* __restric
     win[k] = loader ? src[(int64_t)gy * npsi] : vzero<T>();
     gy = gy + 1 == nzpsi ? 0 : gy + 1;
   }
-  T* dst = out + b * plane;
+  T* dst = tab.out ? tab.out[b] : out + b * plane;
   const int ystop = min(sy, nzpsi - y0);
   const int xr = t / G, xg = t % G;  // x work item: row xr, outputs 8 xg .. 8 xg + 7
   for (int c = 0; c < ystop; c += R) {
@@ -493,7 +493,7 @@ hipError_t bs_launch_dadj_reduce(BsArgs<T> a, const T* g, float2* partial, float
 
 template <typename T>
 hipError_t bs_launch_prefilter(const T* in, T* out, int64_t batch, int nzpsi, int npsi,
-                               const PrefilterTaps& taps, hipStream_t st) {
+                               const PrefilterTaps& taps, hipStream_t st, PlaneTab<T> tab) {
   if (npsi >= kPsTX) {
     // strips of sy rows (a multiple of kPsR): 128 rows (halo overhead 24 / 128)
     // unless that leaves fewer than 256 blocks; measured on 16 and 72 planes of
@@ -505,14 +505,14 @@ hipError_t bs_launch_prefilter(const T* in, T* out, int64_t batch, int nzpsi, in
     const int64_t grid = batch * (int64_t)tiles_x * strips;
     if (grid == 0) return hipSuccess;
     hipLaunchKernelGGL((bs_prefilter_strip_k<T>), dim3((unsigned)grid), dim3(kPsNT), 0, st, in, out, nzpsi, npsi,
-                       tiles_x, strips, sy, taps);
+                       tiles_x, strips, sy, taps, tab);
     return hipGetLastError();
   }
   const int tx = (npsi + kPfTX - 1) / kPfTX, ty = (nzpsi + kPfTY - 1) / kPfTY;
   const int64_t grid = batch * (int64_t)tx * ty;
   if (grid == 0) return hipSuccess;
   hipLaunchKernelGGL((bs_prefilter_k<T>), dim3((unsigned)grid), dim3(256), 0, st, in, out, nzpsi,
-                     npsi, tx, ty, taps);
+                     npsi, tx, ty, taps, tab);
   return hipGetLastError();
 }
 
@@ -522,7 +522,7 @@ hipError_t bs_launch_prefilter(const T* in, T* out, int64_t batch, int nzpsi, in
   template int64_t bs_dadj_partials<T>(const BsArgs<T>&);                                        \
   template hipError_t bs_launch_dadj_reduce<T>(BsArgs<T>, const T*, float2*, float*, hipStream_t); \
   template hipError_t bs_launch_prefilter<T>(const T*, T*, int64_t, int, int, const PrefilterTaps&, \
-                                             hipStream_t);
+                                             hipStream_t, PlaneTab<T>);
 HTAMD_BS_INST(float)
 HTAMD_BS_INST(float2)
 

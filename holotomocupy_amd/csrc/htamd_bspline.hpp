@@ -144,8 +144,15 @@ template <typename T> hipError_t bs_launch_adjoint(BsArgs<T> a, hipStream_t st);
 template <typename T> int64_t bs_dadj_partials(const BsArgs<T>& a);
 template <typename T>
 hipError_t bs_launch_dadj_reduce(BsArgs<T> a, const T* g, float2* partial, float* out2, hipStream_t st);
+// Optional per-plane addresses of the prefilter's input and output (device
+// arrays of batch pointers); a NULL table means plane b at in / out + b * plane.
+template <typename T>
+struct PlaneTab {
+  const T* const* in = nullptr;
+  T* const* out = nullptr;
+};
 template <typename T>
 hipError_t bs_launch_prefilter(const T* in, T* out, int64_t batch, int nzpsi, int npsi,
-                               const PrefilterTaps& taps, hipStream_t st);
+                               const PrefilterTaps& taps, hipStream_t st, PlaneTab<T> tab = PlaneTab<T>());
 
 }  // namespace htamd

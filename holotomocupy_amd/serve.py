@@ -364,10 +364,9 @@ class GpuExecutor:
         torch = self.torch
         cdt = torch.complex64 if geo.obj_dtype == "complex64" else torch.float32
         f32, nd = torch.float32, geo.ndist
-        return [("proj", (A, geo.nzobj, geo.nobj), cdt), ("gproj", (A, geo.nzobj, geo.nobj), cdt),
-                ("pos", (A, nd, 2), f32), ("gpos", (A, nd, 2), f32), ("mag", (A, nd), f32),
-                ("dptr", (A,), torch.int64), ("scales", (A, 4), f32), ("pidx", (A,), torch.int32),
-                ("prb", (P, nd, geo.nz, geo.n), torch.complex64), ("gprb", (P, nd, geo.nz, geo.n), torch.complex64),
+        del cdt
+        return [("pos", (A, nd, 2), f32), ("gpos", (A, nd, 2), f32), ("mag", (A, nd), f32),
+                ("tab", (3 * A + 2 * P,), torch.int64), ("scales", (A, 4), f32), ("pidx", (A,), torch.int32),
                 ("rho_prb", (P,), f32)]
 
     def _gradient_arena(self, si, b, geo, A, P):
@@ -382,9 +381,8 @@ class GpuExecutor:
         return st
 
     def _launch(self, b, st, A, P):
-        b["bh"](st["prb"], st["proj"], st["pos"], None, st["mag"], st["gproj"], st["gpos"], st["gprb"],
-                accumulate_prb=False, probe_index=st["pidx"], scales=st["scales"], rho_prb=st["rho_prb"],
-                workspace=st["ws"], data_ptrs=st["dptr"])
+        b["bh"].call_tab(st["tab"], A, P, st["pos"], st["mag"], st["pidx"], st["scales"], st["rho_prb"], st["gpos"],
+                         st["ws"])
 
     # ---- batches
     def _run_prop(self, key, batch, s):
@@ -480,30 +478,37 @@ class GpuExecutor:
             self._wait_inputs(batch, s)
             st = self._gradient_arena(si, b, geo, A, P)
             mag = b["rec"].eff_demagnifications[0]
+            # the fused passes read each request's projections, data and probe
+            # and write its object and probe gradients where they lie, through
+            # a table of addresses; only the small per-angle arrays are packed
             o = 0
-            # the fused passes read each request's data where it lies, through
-            # a table of per-angle addresses (the data are 90% of a gradient
-            # request's bytes; the other inputs are packed)
-            ptrs = []
+            tab = [[], [], [], [], []]  # proj, data, gproj, prb, gprb
+            outs = []
+            cdt = torch.complex64 if geo.obj_dtype == "complex64" else torch.float32
             for p, r in enumerate(batch):
                 a = r.nangles
-                if not r.data.is_contiguous():
-                    r.data = r.data.contiguous()
+                r.proj = r.proj.to(device=self.device, dtype=cdt).contiguous()
+                r.data = r.data.to(device=self.device, dtype=torch.float32).contiguous()
+                r.prb = r.prb.to(device=self.device, dtype=torch.complex64).contiguous()
                 for t in (r.prb, r.proj, r.pos, r.data):
                     t.record_stream(s)
-                step = r.data[0].numel() * r.data.element_size()
-                ptrs.extend(r.data.data_ptr() + i * step for i in range(a))
-                st["proj"][o:o + a].copy_(r.proj, non_blocking=True)
+                gproj = torch.empty_like(r.proj)
+                gprb = torch.empty_like(r.prb)
+                for t, col in ((r.proj, 0), (r.data, 1), (gproj, 2)):
+                    step = t[0].numel() * t.element_size()
+                    tab[col].extend(t.data_ptr() + i * step for i in range(a))
+                tab[3].append(r.prb.data_ptr())
+                tab[4].append(gprb.data_ptr())
+                outs.append([gprb, gproj, None])
                 st["pos"][o:o + a].copy_(r.pos, non_blocking=True)
                 st["mag"][o:o + a].copy_(mag.expand(a, -1), non_blocking=True)
                 for c, v in enumerate((2.0 / r.data_size, r.rho[0] ** 2, r.rho[2] ** 2, 0.0)):
                     st["scales"][o:o + a, c].fill_(v)  # device fills: no host copies on the stream
                 st["pidx"][o:o + a].fill_(p)
-                st["prb"][p].copy_(r.prb, non_blocking=True)
                 st["rho_prb"][p].fill_(r.rho[1] ** 2)
                 o += a
             # from pinned memory, which the host allocator keeps until the copy has run
-            st["dptr"].copy_(torch.tensor(ptrs, dtype=torch.int64).pin_memory(), non_blocking=True)
+            st["tab"].copy_(torch.tensor(sum(tab, []), dtype=torch.int64).pin_memory(), non_blocking=True)
             key = (geo, A, P)
             g = slot["graphs"].get(key) if self.use_graphs else None
             if g is not None:
@@ -530,11 +535,11 @@ class GpuExecutor:
             else:
                 self._launch(b, st, A, P)
             self.seen[key] += 1
-            outs = []
             o = 0
             for p, r in enumerate(batch):
                 a = r.nangles
-                outs.append((st["gprb"][p].clone(), st["gproj"][o:o + a].clone(), st["gpos"][o:o + a].clone()))
+                outs[p][2] = st["gpos"][o:o + a].clone()
+                outs[p] = tuple(outs[p])
                 o += a
             ev = torch.cuda.Event()
             ev.record(s)

@@ -242,15 +242,19 @@ int htamd_bh_gradient(const htamd_bh_plan* plan, int64_t nangles, const void* pr
                       float* gpos, void* gprb, double* objective, void* workspace,
                       int64_t workspace_bytes, void* stream);
 
-/* As htamd_bh_gradient with the data of angle b read at data_ptrs[b] ([ndist, nz, n] float32 each):
- * data_ptrs is a device array of nangles device pointers, so a batch of requests is served from the
- * requests' own data buffers instead of a packed copy.  No reference counterpart (the reference
- * evaluates one chunk of one array, rec_mpi.py:327-353); it exists for the serving loop. */
-int htamd_bh_gradient_dp(const htamd_bh_plan* plan, int64_t nangles, const void* proj, const float* pos,
-                         const float* mag, const void* prb, const int* probe_index, const float* const* data_ptrs,
-                         const float* angle_scales, const float* rho_prb, int accumulate_prb, void* gproj,
-                         float* gpos, void* gprb, double* objective, void* workspace, int64_t workspace_bytes,
-                         void* stream);
+/* As htamd_bh_gradient with the per-request arrays addressed through a table instead of packed:
+ * tables is a device array of 3 * nangles + 2 * nprobes device pointers,
+ *   [proj of angle 0 .. nangles-1 | data of each angle | gproj of each angle |
+ *    prb of probe slot 0 .. nprobes-1 | gprb of each probe slot],
+ * each pointing at that angle's [nzpsi, npsi] plane ([ndist, nz, n] for data) or that slot's
+ * [ndist, nz, n] probe, so a batch of requests is read from and written to the requests' own
+ * buffers; pos, mag, probe_index, angle_scales, rho_prb and gpos stay packed.  No reference
+ * counterpart (the reference evaluates one chunk of one array, rec_mpi.py:327-353); it exists for
+ * the serving loop. */
+int htamd_bh_gradient_tab(const htamd_bh_plan* plan, int64_t nangles, int64_t nprobes, const void* const* tables,
+                          const float* pos, const float* mag, const int* probe_index, const float* angle_scales,
+                          const float* rho_prb, int accumulate_prb, float* gpos, double* objective, void* workspace,
+                          int64_t workspace_bytes, void* stream);
 
 /* Fused BH bilinear form of the data term.  Replaces Rec.hessian_cascade / _hessian_cascade and the
  * dF1..dF3, d2F_dF0..d2F_dF3 chain (src/holotomocupy/rec_mpi.py:250-303, 517-813) for one chunk of

@@ -81,6 +81,51 @@ def test_fused_gradient_two_probes(gpu):
         assert torch.allclose(gpos[sl], gq, rtol=1e-5, atol=1e-7 * gq.abs().max().item())
 
 
+def test_fused_gradient_address_table_equals_packed(gpu):
+    """htamd_bh_gradient_tab (each angle's projection, data and object
+    gradient and each probe slot's probe and probe gradient addressed through
+    a table, as the serving loop calls it) equals htamd_bh_gradient on the
+    same inputs packed, bit for bit: two requests of 3 and 1 angles with
+    different probes, their arrays in separate allocations."""
+    print("[case] fused gradient address table", flush=True)
+    from holotomocupy_amd.bh import BHGradient
+    rec, cas, x, v, w, d = make_problem(False, seed=43, B=4)
+    bh = BHGradient(rec)
+    xd, dd = dev(x), torch.from_numpy(d).cuda()
+    B = 4
+    prb2 = torch.stack([xd["prb"], xd["prb"] * (0.8 + 0.1j)])
+    pidx = torch.tensor([0, 0, 0, 1], dtype=torch.int32, device="cuda")
+    mag = rec.eff_demagnifications[:B].contiguous()
+    s, rp = bh.scales(B)
+    rho2 = rp.repeat(2)
+    gproj, gpos, gprb = torch.empty_like(xd["proj"]), torch.empty_like(xd["pos"]), torch.empty_like(prb2)
+    bh(prb2, xd["proj"], xd["pos"], dd, mag, gproj, gpos, gprb, accumulate_prb=False, probe_index=pidx, scales=s,
+       rho_prb=rho2)
+    # per-request copies in their own allocations
+    reqs = [(slice(0, 3), 0), (slice(3, 4), 1)]
+    proj_r = [xd["proj"][sl].clone() for sl, _ in reqs]
+    data_r = [dd[sl].clone() for sl, _ in reqs]
+    prb_r = [prb2[p].clone() for _, p in reqs]
+    gproj_r = [torch.full_like(t, float("nan")) for t in proj_r]
+    gprb_r = [torch.full_like(t, float("nan")) for t in prb_r]
+    cols = [[], [], [], [], []]
+    for k in range(2):
+        for t, c in ((proj_r[k], 0), (data_r[k], 1), (gproj_r[k], 2)):
+            step = t[0].numel() * t.element_size()
+            cols[c].extend(t.data_ptr() + i * step for i in range(t.shape[0]))
+        cols[3].append(prb_r[k].data_ptr())
+        cols[4].append(gprb_r[k].data_ptr())
+    tab = torch.tensor(sum(cols, []), dtype=torch.int64, device="cuda")
+    gpos_t = torch.empty_like(gpos)
+    ws = torch.empty(bh.workspace_bytes(B), dtype=torch.uint8, device="cuda")
+    bh.call_tab(tab, B, 2, xd["pos"], mag, pidx, s, rho2, gpos_t, ws)
+    torch.cuda.synchronize()
+    assert torch.equal(gpos_t, gpos)
+    for k, (sl, p) in enumerate(reqs):
+        assert torch.equal(gproj_r[k], gproj[sl]), k
+        assert torch.equal(gprb_r[k], gprb[p]), k
+
+
 def test_fused_gradient_nonsquare_detector(gpu):
     """nz != n and magnified shifts."""
     print("[case] fused gradient non-square", flush=True)
