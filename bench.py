@@ -671,7 +671,7 @@ def op_serve_mix(a, nreq=200, rate=200.0, seed=1005):
     def make(kind, n, nd, A):
         G = geo(n, nd)
         if kind == "op":
-            op = str(rng.choice(["D", "DT", "R", "RT"] if n <= 1024 else ["D", "DT"]))  # Radon plans: n <= 1024
+            op = str(rng.choice(["D", "DT", "R", "RT"]))
             if op in ("R", "RT"):  # slab of 8 slices, A uniform angles on [0, pi), no mask
                 theta = tuple(float(t) for t in np.arange(A) * np.pi / A)
                 shape = (8, n, n) if op == "R" else (A, 8, n)
