@@ -175,7 +175,7 @@ STAGES = ["prefilter", "rows_fwd", "cols", "rows_adj", "sadj", "prefilter_grad",
 # FETCH_SIZE and WRITE_SIZE runs scaled by what the counters read for a 256 MiB
 # device-to-device copy (reads x2.00, writes x1.00); the commands and per-kernel
 # figures are in profiles/bh_gradient_fused_2048_d4_a16_traffic.md.
-TRAFFIC_MIB = {"bh_fwd_rows_k": 1699.3 + 2048.0, "bh_cols_k": 3223.6 + 3380.4, "bh_adj_rows_k": 4004.4 + 2177.3}
+TRAFFIC_MIB = {"bh_fwd_rows_k": 987.4 + 2048.0, "bh_cols_k": 3321.0 + 3303.4, "bh_adj_rows_k": 3995.0 + 2177.3}
 TRAFFIC_WORKLOAD = (2048, 4, 1024, 16)
 
 
@@ -417,11 +417,32 @@ def op_prop(a):
             prop.apply_into(x, y, j, False)
             prop.apply_into(y, y, j, True)
 
-    ms = _timed(step, a.steps, a.warmup)
+    ms_eager = _timed(step, a.steps, a.warmup)
+    ref = y.clone()
+    # the step's 16 launches (eight calls of two passes each) are launch-bound at
+    # 256^2; captured once in a HIP graph they replay as one submission
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        step()
+    torch.cuda.current_stream().wait_stream(side)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        step()
+    for _ in range(3):  # every replay equals the eager step, with eager work in between
+        y.zero_()
+        graph.replay()
+        torch.cuda.synchronize()
+        if not torch.equal(y, ref):
+            raise RuntimeError("graph replay of the D/DT step differs from the eager step")
+        step()
+    ms = _timed(graph.replay, a.steps, a.warmup)
     F = 8 * n * n
     nbytes = nd * 2 * B * 4 * F
     line = _op_line("D+DT field evals/s (16 x 256^2, 4 distances)", nd * 2 * B / (ms * 1e-3), "evals/s", ms,
-                    nbytes, "Fresnel D then DT, 16 fields 256^2, 4 distances", a, {"step": ms})
+                    nbytes, "Fresnel D then DT, 16 fields 256^2, 4 distances (one HIP graph per step)", a,
+                    {"step": ms, "step_eager": ms_eager})
+    line["eager_value"] = round(nd * 2 * B / (ms_eager * 1e-3), 3)
     if not a.no_cpu_baseline:
         from oracle import propagation as op
         o = op.Propagation(n, n, B, nd, 1.0, 1.0, 1.0 / np.array([1e-3, 1.2e-3, 1.5e-3, 2e-3]))
@@ -549,6 +570,40 @@ def op_chunking(a):
     return line
 
 
+def _bilinear_roofline(a, rec, n, nobj, nd, A=16):
+    """The fused bilinear-form passes on one chunk of A angles at the loop's
+    point, in the three forms the loop calls (z = y, general, and the beta
+    step's pair), timed with events.  Model bytes per angle, with NF the
+    propagated fields (3, 4, 5): the prefilters of the 2 or 3 projections
+    (read and write each, 2 F_o apiece), the row pass reading those planes once
+    and writing NF fields per distance, the column pass reading them back with
+    the data (F/2 per distance), and the NF - 1 probes (per distance) read once
+    per launch: 2 NF nd F + nd F / 2 + 3 P F_o + (NF - 1) nd F / A, P the
+    projections (2 or 3)."""
+    import torch
+    F, Fo = 8 * n * n, 8 * nobj * nobj
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    x = {"prb": rec.vars["prb"], "proj": rec.vars["proj"][:A], "pos": rec.vars["pos"][:A]}
+    d = rec.data[:A]
+
+    def direction(sc):
+        return {"prb": sc * 0.1 * rec.vars["prb"],
+                "proj": sc * 0.05 * torch.randn(x["proj"].shape, dtype=x["proj"].dtype, device="cuda", generator=gen),
+                "pos": sc * 0.3 * torch.randn(x["pos"].shape, device="cuda", generator=gen)}
+
+    y, z = direction(1.0), direction(0.7)
+    out = {}
+    for name, nf, nproj, fn in (("same", 3, 2, lambda: rec.hessian_fused(x, y, y, d)),
+                                ("general", 4, 3, lambda: rec.hessian_fused(x, y, z, d)),
+                                ("pair", 5, 3, lambda: rec.hessian_pair_fused(x, y, z, d))):
+        ms = _timed(fn, max(3, a.steps), 1)
+        nbytes = A * (2 * nf * nd * F + nd * F / 2 + 3 * nproj * Fo) + (nf - 1) * nd * F
+        out[name] = {"ms": round(ms, 3), "bytes": int(nbytes), "achieved_gbs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                     "frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    out["angles"] = A
+    return out
+
+
 def op_bh(a):
     """Rec.BH iterations on one GPU (16-angle launches, 16-slice tomography
     chunks, Laplacian regulariser on): `--bh-config c3` is object 512^3, 720
@@ -615,6 +670,7 @@ def op_bh(a):
                     f"Rec.BH: gradient, R of the gradient, three bilinear forms, updates; {shape}, Laplacian on",
                     a, parts)
     line["warmup"] = 1  # the first iteration (no beta step) is the warm-up
+    line["bilinear_roofline"] = _bilinear_roofline(a, rec, n, nobj, nd)
     line["peak_hbm_gib"] = round(torch.cuda.max_memory_allocated() / 2**30, 1)
     line["wall_s_all_iterations"] = round(wall, 3)
     line["alphas"] = [float(x) for x in rec.alphas]
@@ -825,6 +881,13 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
+    if a.copy_calibration:
+        src = torch.ones(1 << 25, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if a.op:
         print(json.dumps({"prop": op_prop, "tomo": op_tomo, "shift": op_shift, "bh": op_bh,
                           "serve-mix": op_serve_mix, "chunking": op_chunking}[a.op](a)), flush=True)
@@ -833,13 +896,6 @@ def main():
     grads = {"prb": torch.zeros_like(vars_["prb"]), "proj": torch.empty_like(vars_["proj"]),
              "pos": torch.empty_like(vars_["pos"])}
     path = "served" if a.path == "auto" else a.path
-    if a.copy_calibration:
-        src = torch.ones(1 << 25, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
     fn_plain = lambda: plain_step(rec, vars_, grads, data)  # noqa: E731
     fused_value = None
     if path == "plain":
