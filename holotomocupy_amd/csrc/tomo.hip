@@ -341,123 +341,140 @@ __global__ __launch_bounds__(256) void tomo_filter_k(const TIO* in, TIO* out, in
 }
 
 // ---- adjoint pass C'': gather onto the grid (transpose of the forward taps) ---
-// Thread per grid point and ZB consecutive slices; the contributions of a
-// grid point are summed in sample order (deterministic, no atomics).  Each
-// CSR entry is one 8-byte (sample, weight) pair, and ZB = 64 (else 32, 16) where the
-// slice count allows, so the index and weight loads are amortised over more slices.
-// Rows through the centre of the grid collect a sample of every angle: at
-// 256^2 with 384 angles the longest has 2005 entries against a mean of 30, and
-// the one thread that owned it set the kernel's duration.  Rows longer than
-// kHeavyRow entries go to tomo_grid_gather_heavy_k instead.
+// Grid point g sums w_k Sh[s_k, z] over its CSR row (packed 8-byte (sample,
+// weight) pairs) in entry order: deterministic, no atomics.  The lanes of a
+// wavefront run across ZB consecutive slices of P = 64 / ZB points, so one
+// entry of a point is one (wave-uniform for ZB = 64) index load and one
+// contiguous ZB x 8-byte read of the sample's slice block -- every byte of a
+// fetched line is used by the instruction that fetched it.  Entries are taken
+// kGatherBatch at a time so that many independent reads are in flight.
+//
+// A workgroup (4 wavefronts) owns PT consecutive grid points and ZB slices;
+// its results go through an LDS tile so the grid is written as rows of PT
+// points per slice (PT = 32, more where a wave step takes 64 / ZB > 8 points;
+// 64 points, LDS-bound at 4 workgroups per CU, measured 9% slower).  Rows through the centre of the grid collect a
+// sample of every angle (at 256^2 with 384 angles the longest has 2005
+// entries against a mean of 30): rows longer than kHeavyRow entries are left
+// to tomo_grid_gather_heavy_k, launched after this kernel on the same stream,
+// which overwrites those points.
 constexpr int kHeavyRow = 64;
+constexpr int kGatherBatch = 8;
+constexpr int kGatherPoints = 32;  // grid points per workgroup of the light rows (at least 4 wave steps)
+constexpr int kHeavyWaves = 4;     // wavefronts per heavy row
+
+template <int ZB>
+constexpr int gather_points() {
+  return kGatherPoints > 4 * (64 / ZB) ? kGatherPoints : 4 * (64 / ZB);
+}
+
+// acc += sum over entries [k0, k1) of w Sh[s, z] in entry order, for the
+// slice block starting at src = Sh + z (zin: this lane's slice exists)
+__device__ __forceinline__ float2 gather_entries(const int2* __restrict__ csw, const float2* __restrict__ src,
+                                                 int64_t nz, int k0, int k1, bool zin, float2 acc) {
+  int k = k0;
+  for (; k + kGatherBatch <= k1; k += kGatherBatch) {
+    int2 cw[kGatherBatch];
+    float2 v[kGatherBatch];
+#pragma unroll
+    for (int u = 0; u < kGatherBatch; ++u) cw[u] = csw[k + u];
+#pragma unroll
+    for (int u = 0; u < kGatherBatch; ++u) v[u] = zin ? src[(int64_t)cw[u].x * nz] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int u = 0; u < kGatherBatch; ++u) acc = vfma(__int_as_float(cw[u].y), v[u], acc);
+  }
+  for (; k < k1; ++k) {
+    const int2 cw = csw[k];
+    const float2 v = zin ? src[(int64_t)cw.x * nz] : make_float2(0.f, 0.f);
+    acc = vfma(__int_as_float(cw.y), v, acc);
+  }
+  return acc;
+}
 
 template <int ZB, bool FULL>
 __global__ __launch_bounds__(256) void tomo_grid_gather_k(TomoArgs a) {
+  constexpr int P = 64 / ZB, PT = gather_points<ZB>(), TS = PT + 1;  // points per wave step, per workgroup; tile stride
+  static_assert(PT % (4 * P) == 0, "whole wave steps per workgroup");
+  __shared__ float2 tile[ZB * TS];
   const int64_t n4 = 4LL * a.n * a.n;
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n4) return;
-  const int k0 = a.rp[g], k1 = a.rp[g + 1];
-  if (k1 - k0 > kHeavyRow) return;  // summed by tomo_grid_gather_heavy_k
+  const int64_t g0 = (int64_t)blockIdx.x * PT;
   const int z0 = blockIdx.y * ZB;
-  float2 acc[ZB];
-#pragma unroll
-  for (int q = 0; q < ZB; ++q) acc[q] = make_float2(0.f, 0.f);
-  for (int k = k0; k < k1; ++k) {
-    const int2 cw = a.csw[k];
-    const float w = __int_as_float(cw.y);
-    const float2* src = a.Sh + (int64_t)cw.x * a.nz + z0;
-    if (FULL) {  // ZB whole slices, 16-byte aligned (nz % ZB == 0): pairs as float4
-      const float4* s4 = (const float4*)src;
-#pragma unroll
-      for (int q = 0; q < ZB / 2; ++q) {
-        const float4 v = s4[q];
-        acc[2 * q] = vfma(w, make_float2(v.x, v.y), acc[2 * q]);
-        acc[2 * q + 1] = vfma(w, make_float2(v.z, v.w), acc[2 * q + 1]);
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < ZB; ++q)
-        if (z0 + q < a.nz) acc[q] = vfma(w, src[q], acc[q]);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
+  const int ps = lane / ZB, q = lane % ZB;
+  const bool zin = FULL || z0 + q < a.nz;
+  const float2* src = a.Sh + z0 + q;
+#pragma unroll 1
+  for (int it = 0; it < PT / 4 / P; ++it) {
+    const int pl = wv * (PT / 4) + it * P + ps;  // point of the tile
+    const int64_t g = g0 + pl;
+    float2 acc = make_float2(0.f, 0.f);
+    if (g < n4) {
+      const int k0 = a.rp[g], k1 = a.rp[g + 1];
+      if (k1 - k0 <= kHeavyRow) acc = gather_entries(a.csw, src, a.nz, k0, k1, zin, acc);
     }
+    tile[q * TS + pl] = acc;
   }
-#pragma unroll
-  for (int q = 0; q < ZB; ++q)
-    if (FULL || z0 + q < a.nz) a.G[(int64_t)(z0 + q) * n4 + g] = acc[q];
+  __syncthreads();
+  // slice r of the tile is PT consecutive grid points, written contiguously
+#pragma unroll 1
+  for (int i = threadIdx.x; i < ZB * PT; i += 256) {
+    const int r = i / PT, pl = i % PT;
+    const int64_t g = g0 + pl;
+    if (g < n4 && (FULL || z0 + r < a.nz)) a.G[(int64_t)(z0 + r) * n4 + g] = tile[r * TS + pl];
+  }
 }
 
-// One wavefront per heavy grid point and ZB slices: lane l sums the entries
-// k0 + l, k0 + l + 64, ... (each in entry order), the 64 partial sums go to
-// LDS, and lane q < ZB adds slice q's partials in lane order -- a fixed
-// summation order, so the result does not depend on scheduling.
+// One workgroup per heavy grid point and ZB slices: its NC = kHeavyWaves P
+// lane groups each sum one contiguous chunk of the point's entries (chunk
+// c = wave P + group, entries [k0 + c L / NC, k0 + (c + 1) L / NC)), in entry order, and lane q
+// then adds the chunks' sums for slice q in chunk order -- a fixed order, so
+// the result does not depend on scheduling.
 template <int ZB, bool FULL>
-__global__ __launch_bounds__(256) void tomo_grid_gather_heavy_k(TomoArgs a) {
-  constexpr int W = 4, PS = ZB + 1;  // waves per workgroup; padded partial row
-  __shared__ float2 part[W * 64 * PS];
-  const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const int hi = blockIdx.x * W + wv;
-  if (hi >= a.nheavy) return;  // the whole wave leaves: no barrier below
+__global__ __launch_bounds__(64 * kHeavyWaves) void tomo_grid_gather_heavy_k(TomoArgs a) {
+  constexpr int P = 64 / ZB, NC = kHeavyWaves * P;
+  __shared__ float2 part[NC * ZB];
   const int64_t n4 = 4LL * a.n * a.n;
-  const int g = a.heavy[hi];
+  const int g = a.heavy[blockIdx.x];
   const int z0 = blockIdx.y * ZB;
-  float2 acc[ZB];
-#pragma unroll
-  for (int q = 0; q < ZB; ++q) acc[q] = make_float2(0.f, 0.f);
-  const int k0 = a.rp[g], k1 = a.rp[g + 1];
-  for (int k = k0 + lane; k < k1; k += 64) {
-    const int2 cw = a.csw[k];
-    const float w = __int_as_float(cw.y);
-    const float2* src = a.Sh + (int64_t)cw.x * a.nz + z0;
-    if (FULL) {
-      const float4* s4 = (const float4*)src;
-#pragma unroll
-      for (int q = 0; q < ZB / 2; ++q) {
-        const float4 v = s4[q];
-        acc[2 * q] = vfma(w, make_float2(v.x, v.y), acc[2 * q]);
-        acc[2 * q + 1] = vfma(w, make_float2(v.z, v.w), acc[2 * q + 1]);
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < ZB; ++q)
-        if (z0 + q < a.nz) acc[q] = vfma(w, src[q], acc[q]);
-    }
-  }
-  float2* mine = part + (wv * 64 + lane) * PS;
-#pragma unroll
-  for (int q = 0; q < ZB; ++q) mine[q] = acc[q];
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if (lane < ZB && (FULL || z0 + lane < a.nz)) {
-    const float2* col = part + wv * 64 * PS + lane;
+  const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int ps = lane / ZB, q = lane % ZB;
+  const bool zin = FULL || z0 + q < a.nz;
+  const int k0 = a.rp[g], L = a.rp[g + 1] - k0;
+  const int c = wv * P + ps;
+  const int kb = k0 + (int)((int64_t)L * c / NC), ke = k0 + (int)((int64_t)L * (c + 1) / NC);
+  part[c * ZB + q] = gather_entries(a.csw, a.Sh + z0 + q, a.nz, kb, ke, zin, make_float2(0.f, 0.f));
+  __syncthreads();
+  if (threadIdx.x < ZB && (FULL || z0 + (int)threadIdx.x < a.nz)) {
     float2 t = make_float2(0.f, 0.f);
-    for (int l = 0; l < 64; ++l) t = cadd(t, col[l * PS]);
-    a.G[(int64_t)(z0 + lane) * n4 + g] = t;
+    for (int cc = 0; cc < NC; ++cc) t = cadd(t, part[cc * ZB + threadIdx.x]);
+    a.G[(int64_t)(z0 + threadIdx.x) * n4 + g] = t;
   }
 }
 
-// The grid gather of the adjoint: thread-per-point rows, then the heavy rows.
-static hipError_t launch_grid_gather(const TomoArgs& a, hipStream_t st) {
+// The grid gather of the adjoint: the rows up to kHeavyRow entries, then the
+// heavy rows.  ZB = 64 slices per workgroup where they divide the slab, else
+// 32, 16, or 8 with the last block masked.
+template <int ZB, bool FULL>
+static void launch_grid_gather_zb(const TomoArgs& a, hipStream_t st) {
   const int64_t n4 = 4LL * a.n * a.n;
-  const unsigned gx = (unsigned)((n4 + 255) / 256), hx = (unsigned)((a.nheavy + 3) / 4);
-  // 64 slices per thread where they divide the slab: half the CSR entry loads
-  // of 32 (RT at 256^3, 384 angles: 3.51 -> 3.27 ms; 16 slices: 3.65 ms)
-  if (a.nz % 64 == 0) {
-    hipLaunchKernelGGL((tomo_grid_gather_k<64, true>), dim3(gx, (unsigned)(a.nz / 64)), dim3(256), 0, st, a);
-    if (a.nheavy)
-      hipLaunchKernelGGL((tomo_grid_gather_heavy_k<32, true>), dim3(hx, (unsigned)(a.nz / 32)), dim3(256), 0, st, a);
-  } else if (a.nz % 32 == 0) {
-    hipLaunchKernelGGL((tomo_grid_gather_k<32, true>), dim3(gx, (unsigned)(a.nz / 32)), dim3(256), 0, st, a);
-    if (a.nheavy)
-      hipLaunchKernelGGL((tomo_grid_gather_heavy_k<32, true>), dim3(hx, (unsigned)(a.nz / 32)), dim3(256), 0, st, a);
-  } else if (a.nz % 16 == 0) {
-    hipLaunchKernelGGL((tomo_grid_gather_k<16, true>), dim3(gx, (unsigned)(a.nz / 16)), dim3(256), 0, st, a);
-    if (a.nheavy)
-      hipLaunchKernelGGL((tomo_grid_gather_heavy_k<16, true>), dim3(hx, (unsigned)(a.nz / 16)), dim3(256), 0, st, a);
-  } else {
-    const unsigned zb = (unsigned)((a.nz + 15) / 16);
-    hipLaunchKernelGGL((tomo_grid_gather_k<16, false>), dim3(gx, zb), dim3(256), 0, st, a);
-    if (a.nheavy) hipLaunchKernelGGL((tomo_grid_gather_heavy_k<16, false>), dim3(hx, zb), dim3(256), 0, st, a);
-  }
+  const unsigned zb = (unsigned)((a.nz + ZB - 1) / ZB);
+  constexpr int PT = gather_points<ZB>();
+  hipLaunchKernelGGL((tomo_grid_gather_k<ZB, FULL>), dim3((unsigned)((n4 + PT - 1) / PT), zb), dim3(256), 0, st, a);
+  if (a.nheavy)
+    hipLaunchKernelGGL((tomo_grid_gather_heavy_k<ZB, FULL>), dim3((unsigned)a.nheavy, zb), dim3(64 * kHeavyWaves), 0,
+                       st, a);
+}
+
+static hipError_t launch_grid_gather(const TomoArgs& a, hipStream_t st) {
+  if (a.nz % 64 == 0)
+    launch_grid_gather_zb<64, true>(a, st);
+  else if (a.nz % 32 == 0)
+    launch_grid_gather_zb<32, true>(a, st);
+  else if (a.nz % 16 == 0)
+    launch_grid_gather_zb<16, true>(a, st);
+  else if (a.nz % 8 == 0)
+    launch_grid_gather_zb<8, true>(a, st);
+  else
+    launch_grid_gather_zb<8, false>(a, st);
   return hipGetLastError();
 }
 
