@@ -220,10 +220,15 @@ __global__ __launch_bounds__(kRowThreads) void bh_fwd_rows_k(BhArgs a) {
 }
 
 // ---- K2: columns: D_y, residual, DT_y ---------------------------------------
-template <int N, int E, int C, int RM = kMaxRadix, bool WAVE = false>
+// The transforms alternate between two LDS tiles (one barrier per FFT stage
+// instead of two, and about 50 fewer registers live at N = 2048): D_y's result
+// lands in the second tile and DT_y's back in the first, from which the tile
+// is stored.  The column's sqrt(data) loads are issued with the tile load, so
+// they land during D_y.
+template <int N, int E, int C>
 __global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_field) {
   constexpr int T = N / E, NT = C * T, LS = col_stride<N, C>();
-  __shared__ float2 lds[C * LS];
+  __shared__ float2 lds[2 * C * LS];
   __shared__ double red[NT / 64 > 0 ? NT / 64 : 1];
   const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const int64_t f = tile / tiles_per_field;  // field b*ndist + j
@@ -231,23 +236,21 @@ __global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_f
   const int64_t b = f / a.ndist;
   const int j = (int)(f % a.ndist);
   const int cc = threadIdx.x / T, t = threadIdx.x % T;
-  FftTw<N, E, RM> twr;
-  fft_load_twiddles<N, E, RM>(twr, t, a.twy);
+  FftTw<N, E> twr;
+  fft_load_twiddles<N, E>(twr, t, a.twy);
   float2* base = a.T + f * (int64_t)N * a.n + x0;
   const float* dbase = (a.data_p ? a.data_p[b] + (int64_t)j * N * a.n : a.data + f * (int64_t)N * a.n) + x0;
   col_tile_xfer<N, E, C, true>(lds, base, a.n, threadIdx.x);  // FFT slot q holds row dct_nat(q)
-  __syncthreads();
-  float2* s = lds + cc * LS;
-  fresnel_1d<N, E, RM, WAVE>(s, t, twr, a.taby + (int64_t)(j * 2 + 0) * 3 * (N / 2));
-  const float sc = a.scales[b].x;
-  double obj = 0.0;
-  int tr = t;
-  asm volatile("" : "+v"(tr));
-  // sqrt(data) of the column, read after the first transform (holding it
-  // across the transform costs registers the wide tiles do not have)
   float dv[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) dv[e] = dbase[dct_nat<N>(tr + e * T) * a.n + cc];
+  for (int e = 0; e < E; ++e) dv[e] = dbase[dct_nat<N>(t + e * T) * a.n + cc];
+  __syncthreads();
+  float2* s0 = lds + cc * LS;
+  float2* s = lds + C * LS + cc * LS;
+  fresnel_1d_pp<N, E>(s0, s, t, twr, a.taby + (int64_t)(j * 2 + 0) * 3 * (N / 2));
+  const float sc = a.scales[b].x;
+  int tr = t;
+  asm volatile("" : "+v"(tr));
   // |x| and d/|x| from one reciprocal square root (v_rsq_f32, ~1 ulp) instead
   // of a correctly rounded square root and division; this thread's E terms of
   // the objective are summed in FP32 and added to the FP64 total once
@@ -263,10 +266,9 @@ __global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_f
     const float g = sc * (1.0f - dv[e] * ri);
     s[p] = make_float2(g * v.x, g * v.y);
   }
-  obj += (double)objf;
-  lds_sync<WAVE>();
-  fresnel_1d<N, E, RM, WAVE>(s, t, twr, a.taby + (int64_t)(j * 2 + 1) * 3 * (N / 2));
-  if constexpr (WAVE) __syncthreads();  // the store reads every wave's column
+  double obj = (double)objf;
+  __syncthreads();
+  fresnel_1d_pp<N, E>(s, s0, t, twr, a.taby + (int64_t)(j * 2 + 1) * 3 * (N / 2));
   int tid = threadIdx.x;
   asm volatile("" : "+v"(tid));
   col_tile_xfer<N, E, C, false>(lds, base, a.n, tid);
@@ -548,7 +550,7 @@ constexpr int bh_col_e() {
 // C = 1 (8-byte rows) takes 5.5 ms against 3.6.
 template <int NZ, int C>
 constexpr bool col_tile_fits() {
-  return C * (NZ / bh_col_e<NZ>()) <= 256 && (int64_t)C * col_stride<NZ, C>() * 8 + 4096 <= 160 * 1024;
+  return C * (NZ / bh_col_e<NZ>()) <= 256 && (int64_t)2 * C * col_stride<NZ, C>() * 8 + 4096 <= 160 * 1024;
 }
 
 template <int NZ>

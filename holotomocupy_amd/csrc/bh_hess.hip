@@ -256,13 +256,41 @@ __global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
 }
 
 // ---- H2: columns ------------------------------------------------------------
+// The fields of a column tile are transformed one after another in two LDS
+// tiles (ping-pong stages: one barrier per FFT stage).  While one field is
+// transformed, the next field's tile is already loading into registers, and
+// sqrt(data) loads with the first field.
 template <int N, int E, int C>
-__device__ __forceinline__ void hcol_transform(float2* lds, const float2* base, int n, int t, int cc,
-                                               const FftTw<N, E>& twr, const float2* tab) {
+struct HColPipe {
+  static constexpr int T = N / E, LS = col_stride<N, C>();
+  float2* A;                 // this thread's column of the tile the fields are loaded into
+  const MakhoulRows<N, E> rows;
+  const int q0;
+  float2 nx[E];              // the next field's tile values (transfer mapping)
+  __device__ __forceinline__ HColPipe(float2* lds, int tid, int n)
+      : A(lds + (tid % C) * LS), rows(tid / C, tid % C, n), q0(tid / C) {}
+  __device__ __forceinline__ void fetch(const float2* base) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) nx[e] = base[rows.off(e)];
+  }
+  // store the fetched tile (FFT slot q holds row dct_nat(q)); the caller
+  // orders it before the transform
+  __device__ __forceinline__ void put() {
+#pragma unroll
+    for (int e = 0; e < E; ++e) A[slot_run<N, E>(q0, e)] = nx[e];
+  }
+};
+
+template <int N, int E, int C>
+__device__ __forceinline__ const float2* hcol_transform_pp(HColPipe<N, E, C>& pp, float2* lds, const float2* next,
+                                                           int t, int cc, const FftTw<N, E>& twr,
+                                                           const float2* tab) {
   constexpr int LS = col_stride<N, C>();
-  col_tile_xfer<N, E, C, true>(lds, const_cast<float2*>(base), n, threadIdx.x);  // slot q: row dct_nat(q)
+  pp.put();
+  if (next) pp.fetch(next);
   __syncthreads();
-  fresnel_1d<N, E>(lds + cc * LS, t, twr, tab);
+  fresnel_1d_pp<N, E>(lds + cc * LS, lds + C * LS + cc * LS, t, twr, tab);
+  return lds + C * LS + cc * LS;
 }
 
 template <int NT>
@@ -295,7 +323,7 @@ __global__ __launch_bounds__(C*(N / E)) void bhh_cols_k(BhhArgs a, int tiles_per
   using HS = HessShape<MODE>;
   constexpr bool SAME = HS::SAME, PAIR = HS::PAIR;
   constexpr int T = N / E, NT = C * T, LS = col_stride<N, C>();
-  __shared__ float2 lds[C * LS];
+  __shared__ float2 lds[2 * C * LS];
   __shared__ double red[NT >= 64 ? NT / 64 : NT];
   const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const int64_t f = tile / tiles_per_field;  // field b*ndist + j
@@ -306,38 +334,36 @@ __global__ __launch_bounds__(C*(N / E)) void bhh_cols_k(BhhArgs a, int tiles_per
   fft_load_twiddles<N, E>(twr, t, a.twy);
   const int64_t off = f * (int64_t)N * a.n + x0c;
   const float2* tab = a.taby + (int64_t)(j * 2 + 0) * 3 * (N / 2);
-  const float2* s = lds + cc * LS;
-  // x0 and sqrt(data) of this thread's elements
-  hcol_transform<N, E, C>(lds, a.F[0] + off, a.n, t, cc, twr, tab);
-  float2 xv[E];
+  HColPipe<N, E, C> pp(lds, threadIdx.x, a.n);
+  // fields in the order x, w, (w of (z, z)), y, (z)
+  pp.fetch(a.F[0] + off);
   float dv[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    xv[e] = s[fpad(t + e * T)];
-    dv[e] = a.data[off + (int64_t)dct_nat<N>(t + e * T) * a.n + cc];
-  }
-  __syncthreads();
+  for (int e = 0; e < E; ++e) dv[e] = a.data[off + (int64_t)dct_nat<N>(t + e * T) * a.n + cc];
+  // x0 of this thread's elements
+  const float2* s = hcol_transform_pp<N, E, C>(pp, lds, a.F[3] + off, t, cc, twr, tab);
+  float2 xv[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) xv[e] = s[fpad(t + e * T)];
   // (1 - d/|x0|) Re(conj(x0) w0), for each form
   double acc = 0.0, acc2 = 0.0;
-  hcol_transform<N, E, C>(lds, a.F[3] + off, a.n, t, cc, twr, tab);
+  s = hcol_transform_pp<N, E, C>(pp, lds, PAIR ? a.F[4] + off : a.F[1] + off, t, cc, twr, tab);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const float2 w = s[fpad(t + e * T)];
     const float am = sqrtf(xv[e].x * xv[e].x + xv[e].y * xv[e].y);
     acc += (double)((1.0f - dv[e] / am) * (xv[e].x * w.x + xv[e].y * w.y));
   }
-  __syncthreads();
   if constexpr (PAIR) {
-    hcol_transform<N, E, C>(lds, a.F[4] + off, a.n, t, cc, twr, tab);
+    s = hcol_transform_pp<N, E, C>(pp, lds, a.F[1] + off, t, cc, twr, tab);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const float2 w = s[fpad(t + e * T)];
       const float am = sqrtf(xv[e].x * xv[e].x + xv[e].y * xv[e].y);
       acc2 += (double)((1.0f - dv[e] / am) * (xv[e].x * w.x + xv[e].y * w.y));
     }
-    __syncthreads();
   }
-  hcol_transform<N, E, C>(lds, a.F[1] + off, a.n, t, cc, twr, tab);
+  s = hcol_transform_pp<N, E, C>(pp, lds, SAME ? nullptr : a.F[2] + off, t, cc, twr, tab);
   if constexpr (SAME) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -352,8 +378,7 @@ __global__ __launch_bounds__(C*(N / E)) void bhh_cols_k(BhhArgs a, int tiles_per
     float2 yv[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) yv[e] = s[fpad(t + e * T)];
-    __syncthreads();
-    hcol_transform<N, E, C>(lds, a.F[2] + off, a.n, t, cc, twr, tab);
+    s = hcol_transform_pp<N, E, C>(pp, lds, nullptr, t, cc, twr, tab);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const float2 zv = s[fpad(t + e * T)];

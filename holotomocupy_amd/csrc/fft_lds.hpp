@@ -2,7 +2,8 @@
 //
 // One transform of length N is carried by T = N/E threads; each thread owns E
 // complex values per stage and performs E/R radix-R butterflies.  The data
-// lives in LDS between stages (in-place, two workgroup barriers per stage).
+// lives in LDS between stages: in place with two workgroup barriers per stage,
+// or (fft_lds_pp) alternating between two buffers with one.
 //
 // LDS layout: element i of a transform buffer lives in slot fpad(i) = i +
 // i/16 (one unused slot after every 16 elements), so the strided writes of the
@@ -234,9 +235,12 @@ __device__ __forceinline__ void tw_powers(float2 w, float2* p) {
   }
 }
 
-// One Stockham stage: span NS (product of radices done so far), radix R.
-template <int N, int E, int R, int NS, bool INV, bool WAVE = false>
-__device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* w) {
+// One Stockham stage: span NS (product of radices done so far), radix R,
+// reading src and writing dst.  In place (src == dst) the reads of every
+// thread must finish before any write (MID: a second ordering point); with two
+// buffers the one ordering point after the writes is enough.
+template <int N, int E, int R, int NS, bool INV, bool WAVE = false, bool MID = true>
+__device__ __forceinline__ void stockham_stage(const float2* src, float2* dst, int t, const float2* w) {
   constexpr int T = N / E;
   constexpr int NB = E / R;
   float2 a[NB][R];
@@ -246,13 +250,13 @@ __device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* w
     if constexpr ((N / R) % 16 == 0) {
       const int pj = fpad(j);  // fpad(j + 16 m) = fpad(j) + 17 m
 #pragma unroll
-      for (int r = 0; r < R; ++r) a[b][r] = s[pj + r * (N / R / 16) * 17];
+      for (int r = 0; r < R; ++r) a[b][r] = src[pj + r * (N / R / 16) * 17];
     } else {
 #pragma unroll
-      for (int r = 0; r < R; ++r) a[b][r] = s[fpad(j + r * (N / R))];
+      for (int r = 0; r < R; ++r) a[b][r] = src[fpad(j + r * (N / R))];
     }
   }
-  lds_sync<WAVE>();
+  if constexpr (MID) lds_sync<WAVE>();
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const int j = t + b * T;
@@ -270,10 +274,10 @@ __device__ __forceinline__ void stockham_stage(float2* s, int t, const float2* w
     if constexpr (NS % 16 == 0) {
       const int pb = fpad(base);
 #pragma unroll
-      for (int r = 0; r < R; ++r) s[pb + r * (NS / 16) * 17] = a[b][r];
+      for (int r = 0; r < R; ++r) dst[pb + r * (NS / 16) * 17] = a[b][r];
     } else {
 #pragma unroll
-      for (int r = 0; r < R; ++r) s[fpad(base + r * NS)] = a[b][r];
+      for (int r = 0; r < R; ++r) dst[fpad(base + r * NS)] = a[b][r];
     }
   }
   lds_sync<WAVE>();
@@ -283,9 +287,28 @@ template <int N, int E, int NS, int OFF, bool INV, int RM, bool WAVE>
 __device__ __forceinline__ void stockham(float2* s, int t, const float2* w) {
   if constexpr (NS < N) {
     constexpr int R = stage_radix<N, E, NS, RM>();
-    stockham_stage<N, E, R, NS, INV, WAVE>(s, t, w + OFF);
+    stockham_stage<N, E, R, NS, INV, WAVE>(s, s, t, w + OFF);
     stockham<N, E, NS * R, OFF + (NS > 1 ? E / R : 0), INV, RM, WAVE>(s, t, w);
   }
+}
+
+// Ping-pong stages: stage i reads buffer i % 2 of (a, b) and writes the other.
+template <int N, int E, int NS, int OFF, bool INV, int RM, bool WAVE, bool FLIP>
+__device__ __forceinline__ void stockham_pp(float2* a, float2* b, int t, const float2* w) {
+  if constexpr (NS < N) {
+    constexpr int R = stage_radix<N, E, NS, RM>();
+    stockham_stage<N, E, R, NS, INV, WAVE, false>(FLIP ? b : a, FLIP ? a : b, t, w + OFF);
+    stockham_pp<N, E, NS * R, OFF + (NS > 1 ? E / R : 0), INV, RM, WAVE, !FLIP>(a, b, t, w);
+  }
+}
+
+// Number of Stockham stages of a length-N transform.
+template <int N, int E, int RM = kMaxRadix, int NS = 1>
+constexpr int fft_stages() {
+  if constexpr (NS >= N)
+    return 0;
+  else
+    return 1 + fft_stages<N, E, RM, NS * stage_radix<N, E, NS, RM>()>();
 }
 
 // Transform the buffer s (fft_buf<N>() slots, element i at s[fpad(i)]) in
@@ -299,6 +322,16 @@ __device__ __forceinline__ void fft_lds(float2* s, int t, const FftTw<N, E, RM>&
   static_assert(E <= N && (E == 2 || E == 4 || E == 8 || E == 16 || E == 32), "E must be 2..32");
   static_assert(!WAVE || N / E == 64, "a wavefront-private transform takes exactly 64 threads");
   stockham<N, E, 1, 0, INV, RM, WAVE>(s, t, tw.w);
+}
+
+// The same transform of a with b as the second buffer: one ordering point per
+// stage instead of two.  The result is in b when fft_stages is odd, else in a.
+template <int N, int E, bool INV, int RM = kMaxRadix, bool WAVE = false>
+__device__ __forceinline__ void fft_lds_pp(float2* a, float2* b, int t, const FftTw<N, E, RM>& tw) {
+  static_assert((N & (N - 1)) == 0 && N >= 2, "N must be a power of two");
+  static_assert(E <= N && (E == 2 || E == 4 || E == 8 || E == 16 || E == 32), "E must be 2..32");
+  static_assert(!WAVE || N / E == 64, "a wavefront-private transform takes exactly 64 threads");
+  stockham_pp<N, E, 1, 0, INV, RM, WAVE, false>(a, b, t, tw.w);
 }
 
 // Makhoul reordering that turns a length-N DCT-II into a length-N FFT:

@@ -45,6 +45,47 @@ __device__ __forceinline__ void fresnel_1d(float2* s, int t, const FftTw<N, E, R
   fft_lds<N, E, true, RM, WAVE>(s, t, tw);
 }
 
+// The pair map from src to dst (each thread reads and writes its own pairs).
+template <int N, int E, bool WAVE = false>
+__device__ __forceinline__ void dct_pair_map_pp(const float2* src, float2* dst, int t,
+                                                const float2* __restrict__ tab) {
+  constexpr int T = N / E;
+  constexpr int H = N / 2;
+#pragma unroll
+  for (int q = 0; q < E / 2; ++q) {
+    const int k = t + q * T;
+    if (k == 0) {
+      dst[0] = cmul(tab[0], src[0]);
+      dst[fpad(H)] = cmul(tab[H], src[fpad(H)]);
+    } else {
+      const int pk = fpad(k), pm = fpad(N - k);
+      const float2 v1 = src[pk], v2 = src[pm];
+      const float2 a = tab[k], b = tab[H + k], c = tab[2 * H + k];
+      dst[pk] = cadd(cmul(a, v1), cmul(b, v2));
+      dst[pm] = cadd(cmul(c, v1), cmul(a, v2));
+    }
+    if (q % 4 == 3) asm volatile("" ::: "memory");
+  }
+  lds_sync<WAVE>();
+}
+
+// fresnel_1d of a with b as the second buffer (one ordering point per FFT
+// stage): the transform, the pair map and the inverse move the data 2 S + 1
+// times (S stages each), an odd count, so the result is always in b.
+template <int N, int E, int RM = kMaxRadix, bool WAVE = false>
+__device__ __forceinline__ void fresnel_1d_pp(float2* a, float2* b, int t, const FftTw<N, E, RM>& tw,
+                                              const float2* __restrict__ tab) {
+  constexpr bool odd = fft_stages<N, E, RM>() % 2 == 1;
+  fft_lds_pp<N, E, false, RM, WAVE>(a, b, t, tw);  // -> odd ? b : a
+  if constexpr (odd) {
+    dct_pair_map_pp<N, E, WAVE>(b, a, t, tab);
+    fft_lds_pp<N, E, true, RM, WAVE>(a, b, t, tw);  // a -> b
+  } else {
+    dct_pair_map_pp<N, E, WAVE>(a, b, t, tab);
+    fft_lds_pp<N, E, true, RM, WAVE>(b, a, t, tw);  // b -> b (even count)
+  }
+}
+
 // Column-pass elements per thread: 32 at N = 2048 (64 threads per column, so a
 // 4-column tile is one 256-thread workgroup), else 8.
 template <int N>
