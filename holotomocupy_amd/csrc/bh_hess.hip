@@ -15,8 +15,11 @@
 // fields and w0 = D(2 py y12 + p w12), as the reference's y_is_z branch.
 //
 // Passes, per (angle, distance):
-//   H1 rows:    staged-row gathers of c, cy, cz with the y weights and their
-//               derivatives, the four (three) input fields, D_x of each  -> fields
+//   H1 chain:   staged-row gathers of c, cy, cz with the y weights and their
+//               derivatives, exp and the probe products: the four (three,
+//               five) input fields                                    -> fields
+//   H1 rows:    D_x of every field in place (the operator's row pass,
+//               one strided batch per distance)
 //   H2 columns: D_y of each field and the pointwise sum; one FP64 partial
 //               per column tile, summed in a fixed order
 // The shift terms are sums over the same 16 taps as S: along y they are
@@ -70,24 +73,25 @@ struct HessShape {
   static constexpr int NOUT = PAIR ? 2 : 1;
 };
 
-// rpb row units per workgroup (rpb * N/E threads, rpb <= kRowThreads / (N/E)),
-// chosen so that the staged rows fit the dynamic LDS
+// H1 is two launches: this kernel evaluates the shift chain per pixel and
+// writes the NF input fields of the row transform (natural order), and the
+// operator's row pass (fresnel_rows_k) then applies D_x to all of them in
+// place.  Nothing is held across transforms, so no per-pixel array is kept
+// (the fused form held four or five complex values per pixel across its NF
+// transforms, in scratch).  A workgroup = rpb row units (j, ty) of T = N/E
+// threads; it loops over the angles.  rpb <= kRowThreads / T, chosen so the
+// staged rows fit the dynamic LDS.
 template <int N, int E, typename TO, int MODE>
-__global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
+__global__ __launch_bounds__(kRowThreads) void bhh_chain_k(BhhArgs a, int rpb) {
   using HS = HessShape<MODE>;
   constexpr bool SAME = HS::SAME, PAIR = HS::PAIR;
-  constexpr int T = N / E, ROWS = kRowThreads / T;
-  constexpr int NR = HS::NR, NF = HS::NF;
-  __shared__ float2 lds[ROWS * fft_buf<N>()];
+  constexpr int T = N / E;
+  constexpr int NR = HS::NR;
   const int r = threadIdx.x / T, t0 = threadIdx.x % T;
   const int64_t nunits = (int64_t)a.ndist * a.nz;
   const int64_t unit = xcd_tile(blockIdx.x, gridDim.x) * rpb + r;
   const bool valid = unit < nunits;
   const int j = valid ? (int)(unit / a.nz) : 0, ty = valid ? (int)(unit % a.nz) : 0;
-  float2* s = lds + r * fft_buf<N>();
-  FftTw<N, E> twr;
-  fft_load_twiddles<N, E>(twr, t0, a.twx);
-  const float2* tab = a.tabx + (int64_t)(j * 2 + 0) * 3 * (N / 2);
   const int64_t plane = (int64_t)a.nzpsi * a.npsi;
   TO* rows = (TO*)dyn_lds() + (int64_t)r * NR * a.npsi;
   TO* R0 = rows;
@@ -153,9 +157,9 @@ __global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
       }
     }
     __syncthreads();
-    // per-pixel values of the second-order chain, kept for the NF transforms
-    float2 ev[E], y12[E], z12[SAME ? 1 : E], w12[E], v12[PAIR ? E : 1];
-    auto chain = [&](int e) {
+    const int64_t orow = (bj * a.nz + ty) * (int64_t)N;
+#pragma unroll 2
+    for (int e = 0; e < E; ++e) {
       const int i = t + e * T;
       const BsPos px = bs_pos(i, N, a.npsi, mb, rx);
       const int ix = px.i - 1;
@@ -186,72 +190,33 @@ __global__ __launch_bounds__(kRowThreads) void bhh_rows_k(BhhArgs a, int rpb) {
       const float2 yv = to_c(vfma(-yyx, sx, ay));
       const float2 wv = to_c(vfma(yyx * zzx, sxx, vfma(1.0f, w1, w0)));
       const float2 ex = expi_fast(to_c(x22));
-      ev[e] = ex;
-      y12[e] = cmul(ex, mul_i(yv));
-      if constexpr (SAME) {
-        const float2 yz = cmul(yv, yv);
-        w12[e] = cmul(ex, make_float2(-wv.y - yz.x, wv.x - yz.y));
-      } else {
-        const float2 zv = to_c(vfma(-zzx, sx, az));
-        z12[e] = cmul(ex, mul_i(zv));
-        const float2 yz = cmul(yv, zv);
-        w12[e] = cmul(ex, make_float2(-wv.y - yz.x, wv.x - yz.y));
-        if constexpr (PAIR) {
-          const float2 vv = to_c(vfma(zzx * zzx, sxx, vfma(1.0f, v1, v0)));
-          const float2 zz = cmul(zv, zv);
-          v12[e] = cmul(ex, make_float2(-vv.y - zz.x, vv.x - zz.y));
-        }
-      }
-    };
-    // equal directions: full unroll keeps the three arrays in registers; the
-    // general and pair forms' four or five arrays measure faster in scratch
-    // with a compact loop (fully unrolled at 2048: 256 VGPRs, one wave per
-    // SIMD, 23.0 -> 27.5 ms for the general form at 16 angles, 4 distances)
-    if constexpr (SAME) {
-#pragma unroll
-      for (int e = 0; e < E; ++e) chain(e);
-    } else {
-#pragma unroll 2
-      for (int e = 0; e < E; ++e) chain(e);
-    }
-    // fields (slots x 0, y 1, z 2, w 3, w of (z, z) 4) in the order w, (wzz,) y, (z,) x
-#pragma unroll 1
-    for (int k = 0; k < NF; ++k) {
-      const int kk = k == 0 ? 3 : k == NF - 1 ? 0 : PAIR ? (k == 1 ? 4 : k - 1) : k;
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int i = t + e * T;
-        const float2 p = a.prb[prow + i];
-        float2 v;
-        if (kk == 0) {
-          v = cmul(p, ev[e]);
-        } else if (kk == 1) {
-          v = cadd(cmul(a.py[prow + i], ev[e]), cmul(p, y12[e]));
-        } else if (kk == 2) {
-          v = cadd(cmul(a.pz[prow + i], ev[e]), cmul(p, z12[SAME ? 0 : e]));
-        } else if (kk == 3) {
-          const float2 pyv = a.py[prow + i];
-          if constexpr (SAME)
-            v = cadd(cscale(cmul(pyv, y12[e]), 2.0f), cmul(p, w12[e]));
-          else
-            v = cadd(cadd(cmul(pyv, z12[e]), cmul(a.pz[prow + i], y12[e])), cmul(p, w12[e]));
-        } else {
-          v = cadd(cscale(cmul(a.pz[prow + i], z12[SAME ? 0 : e]), 2.0f), cmul(p, v12[PAIR ? e : 0]));
-        }
-        s[row_slot<N, E>(t, e)] = v;
-      }
-      __syncthreads();
-      fresnel_1d<N, E>(s, t, twr, tab);
+      const float2 y12 = cmul(ex, mul_i(yv));
+      const float2 p = a.prb[prow + i], pyv = a.py[prow + i];
       if (valid) {
-        float2* dst = a.F[kk] + (bj * a.nz + ty) * (int64_t)N;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-          const int i = t + e * T;
-          dst[i] = s[row_slot<N, E>(t, e)];
+        a.F[0][orow + i] = cmul(p, ex);
+        a.F[1][orow + i] = cadd(cmul(pyv, ex), cmul(p, y12));
+        if constexpr (SAME) {
+          const float2 yz = cmul(yv, yv);
+          const float2 w12 = cmul(ex, make_float2(-wv.y - yz.x, wv.x - yz.y));
+          a.F[3][orow + i] = cadd(cscale(cmul(pyv, y12), 2.0f), cmul(p, w12));
+        } else {
+          const float2 pzv = a.pz[prow + i];
+          const float2 zv = to_c(vfma(-zzx, sx, az));
+          const float2 z12 = cmul(ex, mul_i(zv));
+          const float2 yz = cmul(yv, zv);
+          const float2 w12 = cmul(ex, make_float2(-wv.y - yz.x, wv.x - yz.y));
+          a.F[2][orow + i] = cadd(cmul(pzv, ex), cmul(p, z12));
+          a.F[3][orow + i] = cadd(cadd(cmul(pyv, z12), cmul(pzv, y12)), cmul(p, w12));
+          if constexpr (PAIR) {
+            const float2 vv = to_c(vfma(zzx * zzx, sxx, vfma(1.0f, v1, v0)));
+            const float2 zz = cmul(zv, zv);
+            const float2 v12 = cmul(ex, make_float2(-vv.y - zz.x, vv.x - zz.y));
+            a.F[4][orow + i] = cadd(cscale(cmul(pzv, z12), 2.0f), cmul(p, v12));
+          }
         }
       }
-      __syncthreads();
     }
+    __syncthreads();  // the next angle overwrites the staged rows
   }
 }
 
@@ -520,7 +485,7 @@ static hipError_t hess_rows_m(const BhhArgs& a, hipStream_t st) {
   while (rpb > 1 && rpb * per_row > kHessDynBudget) rpb >>= 1;
   if (rpb * per_row > kHessDynBudget) return hipErrorInvalidConfiguration;
   const int64_t grid = ((int64_t)a.ndist * a.nz + rpb - 1) / rpb;
-  hipLaunchKernelGGL((bhh_rows_k<N, E, TO, MODE>), dim3((unsigned)grid), dim3(rpb * T), (size_t)(rpb * per_row),
+  hipLaunchKernelGGL((bhh_chain_k<N, E, TO, MODE>), dim3((unsigned)grid), dim3(rpb * T), (size_t)(rpb * per_row),
                      st, a, rpb);
   return hipGetLastError();
 }
@@ -594,6 +559,21 @@ static hipError_t bhh_run(const htamd_shift_plan* sp, int mode, BhhArgs a, const
           hipSuccess)
     return e;
   if ((e = hess_rows_dispatch<TO>(mode, a, st)) != hipSuccess) return e;
+  // D_x of the NF input fields, one strided batch per distance: the used field
+  // slots are consecutive (bhh_layout), field (k, b, j) at F[0] + ((k nang + b)
+  // ndist + j) nz n
+  const int nf = mode == kSame ? 3 : mode == kPair ? 5 : 4;
+  const int64_t field = (int64_t)a.n * a.nz;
+  const int64_t slot = a.nang * a.ndist * field;
+  const int used[5] = {0, 1, mode == kSame ? 3 : 2, 3, 4};
+  for (int k = 0; k < nf; ++k)
+    if (a.F[used[k]] != a.F[0] + k * slot) return hipErrorInvalidValue;  // slots not consecutive
+  for (int j = 0; j < a.ndist; ++j) {
+    float2* f0 = a.F[0] + j * field;
+    if ((e = launch_fresnel_rows(a.n, a.nz, f0, a.ndist * field, f0, a.ndist * field, nf * a.nang, a.twx,
+                                 a.tabx + (int64_t)(j * 2 + 0) * 3 * (a.n / 2), st)) != hipSuccess)
+      return e;
+  }
   if ((e = hess_cols_dispatch(mode, a, st)) != hipSuccess) return e;
   const int64_t ntiles = a.nang * a.ndist * (a.n / hess_cols_c(a.nz));
   const int nout = mode == kPair ? 2 : 1;
