@@ -108,6 +108,77 @@ __global__ __launch_bounds__(kTX* kTY) void bs_forward_k(BsArgs<T> a) {
   a.out[(b * a.nz + ty) * (int64_t)a.n + tx] = acc;
 }
 
+// S with a wavefront per 64 columns x kSRY consecutive output rows.  The rows'
+// y windows overlap (three of four taps at magnification 1), so the wave walks
+// the coefficient rows its outputs need in ascending order, forms each row's
+// x-tap sum once and adds it to every output whose window holds that row --
+// the same sums in the same order as bs_forward_k<.., BS_S>, with about
+// (kSRY + 3) / kSRY row reads per output instead of four.
+constexpr int kSRY = 8;
+
+template <typename T>
+__global__ __launch_bounds__(256) void bs_interp_rows_k(BsArgs<T> a) {
+  constexpr int RY = kSRY;
+  const int64_t bid = blockIdx.x;
+  const int txt = (int)(bid % a.tiles_x);
+  const int64_t rest = bid / a.tiles_x;
+  const int tyt = (int)(rest % a.tiles_y);
+  const int64_t b = rest / a.tiles_y;
+  const int tx = txt * 64 + (int)(threadIdx.x % 64);
+  const int ty0 = tyt * (4 * RY) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64)) * RY;
+  if (ty0 >= a.nz) return;  // the whole wave
+  const float mb = a.m[b * a.ms];
+  const BsPos px = bs_pos(tx, a.n, a.npsi, mb, a.r[b * a.rs + 1]);
+  const int ix = px.i - 1;
+  float wx[4];
+  bs_w(px.d, wx);
+  const float ry = a.r[b * a.rs + 0];
+  int iy[RY];
+  float wy[RY][4];
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const BsPos py = bs_pos(ty0 + r, a.nz, a.nzpsi, mb, ry);
+    iy[r] = py.i - 1;
+    bs_w(py.d, wy[r]);
+  }
+  const T* c = a.c + b * (int64_t)a.nzpsi * a.npsi;
+  T acc[RY];
+#pragma unroll
+  for (int r = 0; r < RY; ++r) acc[r] = vzero<T>();
+  // rows past nz are evaluated like the others and not stored; their windows
+  // only extend the walk
+  int ylo = iy[0], yhi = iy[0];
+#pragma unroll
+  for (int r = 1; r < RY; ++r) {
+    ylo = min(ylo, iy[r]);
+    yhi = max(yhi, iy[r]);
+  }
+  ylo = max(ylo, 0);
+  yhi = min(yhi + 3, a.nzpsi - 1);
+  for (int yy = ylo; yy <= yhi; ++yy) {
+    T row = vzero<T>();
+#pragma unroll
+    for (int jx = 0; jx < 4; ++jx) {
+      const int xx = ix + jx;
+      if (xx < 0 || xx >= a.npsi) continue;
+      row = vfma(wx[jx], c[(int64_t)yy * a.npsi + xx], row);
+    }
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const int d = yy - iy[r];
+      if (d >= 0 && d < 4) {
+        const float w = d == 0 ? wy[r][0] : d == 1 ? wy[r][1] : d == 2 ? wy[r][2] : wy[r][3];
+        acc[r] = vfma(w, row, acc[r]);
+      }
+    }
+  }
+  if (tx < a.n) {
+#pragma unroll
+    for (int r = 0; r < RY; ++r)
+      if (ty0 + r < a.nz) a.out[(b * a.nz + ty0 + r) * (int64_t)a.n + tx] = acc[r];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Sadj as a gather over the coefficient grid: c[b, iy, ix] =
 //   sum_{ty: iy in taps(y(ty))} wy(ty) sum_{tx: ix in taps(x(tx))} wx(tx) g[b, ty, tx].
@@ -163,19 +234,35 @@ __global__ __launch_bounds__(256) void bs_adjoint_k(BsArgs<T> a) {
 #pragma unroll
       for (int k = 0; k < K; ++k)
         wxs[k] = xlo + k <= xhi ? bs_reach(bs_pos(xlo + k, a.n, a.npsi, mb, rx), ix) : 0.0f;
+      // the R outputs' candidate detector rows overlap (three of four at
+      // magnification 1): walk their union in ascending order, form each
+      // row's x sum once and add it to every output that weights it -- per
+      // output the same terms in the same order as a walk of its own rows
+      int ylo = s_ylo[r0], yhi = s_ylo[r0];
 #pragma unroll
-      for (int rr = 0; rr < R; ++rr) {
-        const int ylo = s_ylo[r0 + rr];
-        for (int k = 0; k < K; ++k) {
-          const float wyv = s_wy[r0 + rr][k];
-          if (wyv == 0.0f) continue;  // wave-uniform: the row is the wave's
-          const T* gr = g + (int64_t)(ylo + k) * a.n + xlo;
-          T row = vzero<T>();
+      for (int rr = 1; rr < R; ++rr) {
+        ylo = min(ylo, s_ylo[r0 + rr]);
+        yhi = max(yhi, s_ylo[r0 + rr]);
+      }
+      yhi += K - 1;
+      for (int ty = ylo; ty <= yhi; ++ty) {
+        float wv[R];
+        bool any = false;
 #pragma unroll
-          for (int kk = 0; kk < K; ++kk)
-            if (wxs[kk] != 0.0f) row = vfma(wxs[kk], gr[kk], row);
-          acc[rr] = vfma(wyv, row, acc[rr]);
+        for (int rr = 0; rr < R; ++rr) {
+          const int d = ty - s_ylo[r0 + rr];
+          wv[rr] = d >= 0 && d < K ? s_wy[r0 + rr][d] : 0.0f;
+          any = any || wv[rr] != 0.0f;
         }
+        if (!any) continue;  // wave-uniform: the rows are the wave's
+        const T* gr = g + (int64_t)ty * a.n + xlo;
+        T row = vzero<T>();
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk)
+          if (wxs[kk] != 0.0f) row = vfma(wxs[kk], gr[kk], row);
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr)
+          if (wv[rr] != 0.0f) acc[rr] = vfma(wv[rr], row, acc[rr]);
       }
     }
   } else if (xin) {
@@ -446,12 +533,18 @@ static void tiles_of(BsArgs<T>& a, bool coeff_grid) {
 
 template <typename T>
 hipError_t bs_launch_forward(BsArgs<T> a, int mode, hipStream_t st) {
+  if (mode == BS_S) {
+    a.tiles_x = (a.n + 63) / 64;
+    a.tiles_y = (a.nz + 4 * kSRY - 1) / (4 * kSRY);
+    const int64_t grid = a.batch * (int64_t)a.tiles_x * a.tiles_y;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL((bs_interp_rows_k<T>), dim3((unsigned)grid), dim3(256), 0, st, a);
+    return hipGetLastError();
+  }
   tiles_of(a, false);
   const int64_t grid = a.batch * (int64_t)a.tiles_x * a.tiles_y;
   if (grid == 0) return hipSuccess;
-  if (mode == BS_S)
-    hipLaunchKernelGGL((bs_forward_k<T, BS_S>), dim3((unsigned)grid), dim3(kTX * kTY), 0, st, a);
-  else if (mode == BS_D1)
+  if (mode == BS_D1)
     hipLaunchKernelGGL((bs_forward_k<T, BS_D1>), dim3((unsigned)grid), dim3(kTX * kTY), 0, st, a);
   else
     hipLaunchKernelGGL((bs_forward_k<T, BS_D2>), dim3((unsigned)grid), dim3(kTX * kTY), 0, st, a);
