@@ -440,19 +440,34 @@ __global__ __launch_bounds__(256) void bh_sadj_k(BhArgs a, TO* __restrict__ dpsi
 #pragma unroll
         for (int k = 0; k < K; ++k)
           wxs[k] = xlo + k <= xhi ? bs_reach(bs_pos(xlo + k, a.n, a.npsi, mb, rx), ix) : 0.0f;
+        // the R outputs' candidate detector rows overlap: walk their union in
+        // ascending order, form each row's x sum once and add it to every
+        // output that weights it (per output the same terms in the same order)
+        int ylo = s_ylo[r0], yhi = s_ylo[r0];
 #pragma unroll
-        for (int rr = 0; rr < R; ++rr) {
-          const int ylo = s_ylo[r0 + rr];
-          for (int k = 0; k < K; ++k) {
-            const float wyv = s_wy[r0 + rr][k];
-            if (wyv == 0.0f) continue;  // wave-uniform: the row is the wave's
-            const float2* gr = g + (int64_t)(ylo + k) * a.n + xlo;
-            TO rowv = vzero<TO>();
+        for (int rr = 1; rr < R; ++rr) {
+          ylo = min(ylo, s_ylo[r0 + rr]);
+          yhi = max(yhi, s_ylo[r0 + rr]);
+        }
+        yhi += K - 1;
+        for (int ty = ylo; ty <= yhi; ++ty) {
+          float wv[R];
+          bool any = false;
 #pragma unroll
-            for (int kk = 0; kk < K; ++kk)
-              if (wxs[kk] != 0.0f) rowv = vfma(wxs[kk], from_c<TO>(gr[kk]), rowv);
-            acc[rr] = vfma(wyv, rowv, acc[rr]);
+          for (int rr = 0; rr < R; ++rr) {
+            const int d = ty - s_ylo[r0 + rr];
+            wv[rr] = d >= 0 && d < K ? s_wy[r0 + rr][d] : 0.0f;
+            any = any || wv[rr] != 0.0f;
           }
+          if (!any) continue;  // wave-uniform: the rows are the wave's
+          const float2* gr = g + (int64_t)ty * a.n + xlo;
+          TO rowv = vzero<TO>();
+#pragma unroll
+          for (int kk = 0; kk < K; ++kk)
+            if (wxs[kk] != 0.0f) rowv = vfma(wxs[kk], from_c<TO>(gr[kk]), rowv);
+#pragma unroll
+          for (int rr = 0; rr < R; ++rr)
+            if (wv[rr] != 0.0f) acc[rr] = vfma(wv[rr], rowv, acc[rr]);
         }
       }
     } else if (xin) {
