@@ -206,11 +206,18 @@ __global__ __launch_bounds__(256) void tomo_tile_gather_k(TomoArgs a) {
   const int n2 = 2 * a.n, P = a.ts + kTL - 1;
   const int64_t plane = (int64_t)n2 * n2;
   const int gy0 = it.y * a.ts, gx0 = it.x * a.ts;
-  for (int idx = threadIdx.x; idx < kTZ * P * P; idx += 256) {
-    const int q = idx / (P * P), rem = idx % (P * P);
-    const int r = rem / P, c = rem % P;
+  // staged element (q, r, c) by the compile-time pitch kTP (the divisions are
+  // multiply-shifts); the wrap onto the 2n grid is one conditional subtraction:
+  // the tile origin is a multiple of ts below 2n and r, c < ts + 8, which keeps
+  // origin + r below 4n for every even n >= 8
+  for (int idx = threadIdx.x; idx < kTZ * kTP * kTP; idx += 256) {
+    const int q = idx / (kTP * kTP), rem = idx - q * (kTP * kTP);
+    const int r = rem / kTP, c = rem - r * kTP;
+    if (r >= P || c >= P) continue;  // tiles narrower than kTS (2n < kTS)
     const int z = z0 + q;
-    const int gr = (gy0 + r) % n2, gc = (gx0 + c) % n2;
+    int gr = gy0 + r, gc = gx0 + c;
+    gr -= gr >= n2 ? n2 : 0;
+    gc -= gc >= n2 ? n2 : 0;
     tile[q * kTP * kTP + r * kTP + c] = z < a.nz ? a.G[z * plane + (int64_t)gr * n2 + gc] : make_float2(0.f, 0.f);
   }
   __syncthreads();
