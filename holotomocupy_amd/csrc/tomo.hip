@@ -200,25 +200,30 @@ __global__ __launch_bounds__(256) void tomo_gather_k(TomoArgs a, TO* __restrict_
 // line spectra Sh[(theta nz + z) n + t], scaled by c1d / n.
 constexpr int kTS = 32, kTL = 9, kTP = kTS + kTL - 1, kTZ = 4;
 __global__ __launch_bounds__(256) void tomo_tile_gather_k(TomoArgs a) {
-  __shared__ float2 tile[kTZ * kTP * kTP];
+  static_assert(kTZ == 4, "a staged point's slices are two float4");
+  // slice-minor: point (r, c) holds its kTZ slices at tile[(r kTP + c) kTZ + q],
+  // so a tap's four slices are two 16-byte LDS reads
+  __shared__ float4 tile4[kTP * kTP * kTZ / 2];
+  float2* tile = reinterpret_cast<float2*>(tile4);
   const int4 it = a.items[blockIdx.x];
   const int z0 = blockIdx.y * kTZ;
   const int n2 = 2 * a.n, P = a.ts + kTL - 1;
   const int64_t plane = (int64_t)n2 * n2;
   const int gy0 = it.y * a.ts, gx0 = it.x * a.ts;
   // staged element (q, r, c) by the compile-time pitch kTP (the divisions are
-  // multiply-shifts); the wrap onto the 2n grid is one conditional subtraction:
-  // the tile origin is a multiple of ts below 2n and r, c < ts + 8, which keeps
+  // multiply-shifts), slice fastest so consecutive threads fill consecutive
+  // LDS slots; the wrap onto the 2n grid is one conditional subtraction: the
+  // tile origin is a multiple of ts below 2n and r, c < ts + 8, which keeps
   // origin + r below 4n for every even n >= 8
   for (int idx = threadIdx.x; idx < kTZ * kTP * kTP; idx += 256) {
-    const int q = idx / (kTP * kTP), rem = idx - q * (kTP * kTP);
-    const int r = rem / kTP, c = rem - r * kTP;
+    const int pt = idx / kTZ, q = idx - pt * kTZ;
+    const int r = pt / kTP, c = pt - r * kTP;
     if (r >= P || c >= P) continue;  // tiles narrower than kTS (2n < kTS)
     const int z = z0 + q;
     int gr = gy0 + r, gc = gx0 + c;
     gr -= gr >= n2 ? n2 : 0;
     gc -= gc >= n2 ? n2 : 0;
-    tile[q * kTP * kTP + r * kTP + c] = z < a.nz ? a.G[z * plane + (int64_t)gr * n2 + gc] : make_float2(0.f, 0.f);
+    tile[idx] = z < a.nz ? a.G[z * plane + (int64_t)gr * n2 + gc] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   if ((int)threadIdx.x >= it.w) return;
@@ -226,29 +231,39 @@ __global__ __launch_bounds__(256) void tomo_tile_gather_k(TomoArgs a) {
   const int2 so = a.so[k];
   const float2 f = a.sf[k];
   const int ox = so.x & 0xffff, oy = so.x >> 16, si = so.y;
-  float wx[kTL], wy[kTL];
+  float wx[kTL];
 #pragma unroll
   for (int i = 0; i < kTL; ++i) {
-    const float dx = (float)(i - a.m) - f.x, dy = (float)(i - a.m) - f.y;
+    const float dx = (float)(i - a.m) - f.x;
     wx[i] = expf(a.cxs * dx * dx);
-    wy[i] = a.cys * expf(a.cxs * dy * dy);
   }
   const int th = si / a.n, t = si % a.n;
   const float sc = -sgnf(t) / (float)a.n;
-  for (int q = 0; q < kTZ; ++q) {
-    const int z = z0 + q;
-    if (z >= a.nz) break;
-    const float2* tq = tile + q * kTP * kTP + oy * kTP + ox;
-    float2 acc = make_float2(0.f, 0.f);
+  const float4* tq = tile4 + (oy * kTP + ox) * 2;
+  float2 acc[kTZ];
 #pragma unroll
-    for (int i1 = 0; i1 < kTL; ++i1) {
-      float2 racc = make_float2(0.f, 0.f);
+  for (int q = 0; q < kTZ; ++q) acc[q] = make_float2(0.f, 0.f);
+#pragma unroll 1
+  for (int i1 = 0; i1 < kTL; ++i1) {  // one tap row at a time (registers)
+    const float dy = (float)(i1 - a.m) - f.y;
+    const float wy = a.cys * expf(a.cxs * dy * dy);
+    float2 racc[kTZ];
 #pragma unroll
-      for (int i0 = 0; i0 < kTL; ++i0) racc = vfma(wx[i0], tq[i1 * kTP + i0], racc);
-      acc = vfma(wy[i1], racc, acc);
+    for (int q = 0; q < kTZ; ++q) racc[q] = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int i0 = 0; i0 < kTL; ++i0) {
+      const float4 v01 = tq[(i1 * kTP + i0) * 2], v23 = tq[(i1 * kTP + i0) * 2 + 1];
+      racc[0] = vfma(wx[i0], make_float2(v01.x, v01.y), racc[0]);
+      racc[1] = vfma(wx[i0], make_float2(v01.z, v01.w), racc[1]);
+      racc[2] = vfma(wx[i0], make_float2(v23.x, v23.y), racc[2]);
+      racc[3] = vfma(wx[i0], make_float2(v23.z, v23.w), racc[3]);
     }
-    a.Sh[((int64_t)th * a.nz + z) * a.n + t] = cscale(acc, sc);
+#pragma unroll
+    for (int q = 0; q < kTZ; ++q) acc[q] = vfma(wy, racc[q], acc[q]);
   }
+#pragma unroll
+  for (int q = 0; q < kTZ; ++q)
+    if (z0 + q < a.nz) a.Sh[((int64_t)th * a.nz + z0 + q) * a.n + t] = cscale(acc[q], sc);
 }
 
 // ---- forward pass D: line IFFT of the gathered spectra ----------------------
