@@ -62,6 +62,27 @@ def test_S_Sadj_coeff(gpu, geom, real):
     assert rel(g.curlyS(dev(c), dev(r), dev(m)).cpu(), o.curlyS(c, r, m)) < REL_TOL
 
 
+@pytest.mark.parametrize("real", [False, True])
+def test_S_Sadj_wide_magnifications(gpu, real):
+    """S walks the coefficient rows of 8 output rows at once and Sadj the union
+    of 4 outputs' candidate detector rows: magnifications far from 1 widen
+    those walks (2.3: 8-row windows spread over ~20 coefficient rows; 0.45:
+    many outputs share a row), and 70 and 20 detector rows end mid-block."""
+    from holotomocupy_amd import Shift
+    print(f"[case] S/Sadj wide magnifications real={real}", flush=True)
+    rng = np.random.default_rng(15)
+    dt = "float32" if real else "complex64"
+    for n, npsi, nz, nzpsi in ((64, 160, 70, 170), (96, 48, 20, 40)):
+        B = 4
+        g, o = Shift(n, npsi, nz, nzpsi, dt), osh.Shift(n, npsi, nz, nzpsi, dt)
+        r = rng.uniform(-4, 4, (B, 2)).astype(np.float32)
+        m = np.array([0.45, 1.0, 1.7, 2.3], dtype=np.float32)
+        c = data(rng, (B, nzpsi, npsi), real)
+        s = data(rng, (B, nz, n), real)
+        assert rel(g.S(dev(c), dev(r), dev(m)).cpu(), o.S(c, r, m)) < REL_TOL, (n, npsi, nz, nzpsi)
+        assert rel(g.Sadj(dev(s), dev(r), dev(m)).cpu(), o.Sadj(s, r, m)) < REL_TOL, (n, npsi, nz, nzpsi)
+
+
 @pytest.mark.parametrize("geom", GEOMS[:3])
 @pytest.mark.parametrize("real", [False, True])
 def test_derivatives(gpu, geom, real):
