@@ -15,6 +15,8 @@
 //               y22 and the shift-gradient partial sums            -> T
 //   K4 object:  gather-form Sadj summed over distances              -> dpsi
 // then the B-spline prefilter of dpsi and two small deterministic reductions.
+#include <climits>
+
 #include "bh_common.hpp"
 
 namespace htamd {
@@ -443,9 +445,11 @@ __global__ __launch_bounds__(256) void bh_sadj_k(BhArgs a, TO* __restrict__ dpsi
         // the R outputs' candidate detector rows overlap: walk their union in
         // ascending order, form each row's x sum once and add it to every
         // output that weights it (per output the same terms in the same order)
-        int ylo = s_ylo[r0], yhi = s_ylo[r0];
+        // over the rows that exist (rows past the plane have no candidates)
+        int ylo = INT_MAX, yhi = INT_MIN;
 #pragma unroll
-        for (int rr = 1; rr < R; ++rr) {
+        for (int rr = 0; rr < R; ++rr) {
+          if (tyt * TY + r0 + rr >= a.nzpsi) continue;
           ylo = min(ylo, s_ylo[r0 + rr]);
           yhi = max(yhi, s_ylo[r0 + rr]);
         }

@@ -6,6 +6,8 @@
 // Sampling point of output pixel (tx, ty) of angle b (cuda_kernels.py:172-181):
 //   x = (m_b (tx - n/2) - r_b[1] + (m_b - 1)/2) + npsi/2      (same for y, r_b[0])
 // taps floor(x) + j, j = -1..2, weights phi(dx - j), phi = 6 * cubic B-spline.
+#include <climits>
+
 #include "htamd_bspline.hpp"
 #include "htamd_internal.hpp"
 
@@ -238,9 +240,11 @@ __global__ __launch_bounds__(256) void bs_adjoint_k(BsArgs<T> a) {
       // magnification 1): walk their union in ascending order, form each
       // row's x sum once and add it to every output that weights it -- per
       // output the same terms in the same order as a walk of its own rows
-      int ylo = s_ylo[r0], yhi = s_ylo[r0];
+      // over the rows that exist (rows past the plane have no candidates)
+      int ylo = INT_MAX, yhi = INT_MIN;
 #pragma unroll
-      for (int rr = 1; rr < R; ++rr) {
+      for (int rr = 0; rr < R; ++rr) {
+        if (tyt * TY + r0 + rr >= a.nzpsi) continue;
         ylo = min(ylo, s_ylo[r0 + rr]);
         yhi = max(yhi, s_ylo[r0 + rr]);
       }
