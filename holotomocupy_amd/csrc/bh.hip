@@ -157,8 +157,12 @@ __device__ __forceinline__ float2 row_sum2(float2 v, float2* red) {
 
 // ---- K1: rows, forward ------------------------------------------------------
 // Workgroup = ROWS row units (j, ty); loops over the angles of the launch.
+// Four waves per SIMD (at most 128 VGPRs): the staged gather waits on memory
+// for about half of its cycles, and the allocator's free choice at N = 2048
+// is 129 registers, i.e. three waves (1.74 against 1.95 ms per 16 angles x 4
+// distances at 2048^2, no spills at any N).
 template <int N, int E, typename TO, bool STAGED>
-__global__ __launch_bounds__(kRowThreads) void bh_fwd_rows_k(BhArgs a) {
+__global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(4))) void bh_fwd_rows_k(BhArgs a) {
   constexpr int T = N / E, ROWS = kRowThreads / T;
   __shared__ float2 lds[ROWS * fft_buf<N>()];
   const int r = threadIdx.x / T, t0 = threadIdx.x % T;
