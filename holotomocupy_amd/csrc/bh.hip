@@ -225,12 +225,35 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(4))
   }
 }
 
+// K2's residual 2/N (x - d x/|x|) and the objective term (|x| - d)^2 of one
+// element, applied to D_y's result as the last inverse stage writes it (output
+// e of a thread is element t + e N/E, whose sqrt(data) the thread holds in
+// dv[e]).  |x| and d/|x| come from one reciprocal square root (v_rsq_f32,
+// ~1 ulp) instead of a correctly rounded square root and division; a thread's
+// E objective terms are summed in FP32 and added to the FP64 total once.
+template <int E>
+struct BhResidual {
+  static constexpr bool active = true, store = true;
+  const float (&dv)[E];
+  float sc;
+  float& objf;
+  __device__ __forceinline__ float2 operator()(int e, float2 v) const {
+    const float a2 = v.x * v.x + v.y * v.y;
+    const float ri = __builtin_amdgcn_rsqf(a2);
+    const float q = a2 * ri - dv[e];
+    objf = fmaf(q, q, objf);
+    const float g = sc * (1.0f - dv[e] * ri);
+    return make_float2(g * v.x, g * v.y);
+  }
+};
+
 // ---- K2: columns: D_y, residual, DT_y ---------------------------------------
 // The transforms alternate between two LDS tiles (one barrier per FFT stage
 // instead of two, and about 50 fewer registers live at N = 2048): D_y's result
 // lands in the second tile and DT_y's back in the first, from which the tile
 // is stored.  The column's sqrt(data) loads are issued with the tile load, so
-// they land during D_y.
+// they land during D_y, and the residual is formed in D_y's last stage
+// (BhResidual) rather than in a pass over the tile of its own.
 template <int N, int E, int C>
 __global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_field) {
   constexpr int T = N / E, NT = C * T, LS = col_stride<N, C>();
@@ -253,27 +276,12 @@ __global__ __launch_bounds__(C*(N / E)) void bh_cols_k(BhArgs a, int tiles_per_f
   __syncthreads();
   float2* s0 = lds + cc * LS;
   float2* s = lds + C * LS + cc * LS;
-  fresnel_1d_pp<N, E>(s0, s, t, twr, a.taby + (int64_t)(j * 2 + 0) * 3 * (N / 2));
   const float sc = a.scales[b].x;
-  int tr = t;
-  asm volatile("" : "+v"(tr));
-  // |x| and d/|x| from one reciprocal square root (v_rsq_f32, ~1 ulp) instead
-  // of a correctly rounded square root and division; this thread's E terms of
-  // the objective are summed in FP32 and added to the FP64 total once
   float objf = 0.0f;
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int p = fpad(tr + e * T);
-    const float2 v = s[p];
-    const float a2 = v.x * v.x + v.y * v.y;
-    const float ri = __builtin_amdgcn_rsqf(a2);
-    const float q = a2 * ri - dv[e];
-    objf = fmaf(q, q, objf);
-    const float g = sc * (1.0f - dv[e] * ri);
-    s[p] = make_float2(g * v.x, g * v.y);
-  }
+  const BhResidual<E> res{dv, sc, objf};
+  fresnel_1d_pp<N, E, kMaxRadix, false, BhResidual<E>>(s0, s, t, twr, a.taby + (int64_t)(j * 2 + 0) * 3 * (N / 2),
+                                                       res);
   double obj = (double)objf;
-  __syncthreads();
   fresnel_1d_pp<N, E>(s, s0, t, twr, a.taby + (int64_t)(j * 2 + 1) * 3 * (N / 2));
   int tid = threadIdx.x;
   asm volatile("" : "+v"(tid));

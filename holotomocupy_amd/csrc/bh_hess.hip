@@ -246,17 +246,84 @@ struct HColPipe {
   }
 };
 
-template <int N, int E, int C>
-__device__ __forceinline__ const float2* hcol_transform_pp(HColPipe<N, E, C>& pp, float2* lds, const float2* next,
-                                                           int t, int cc, const FftTw<N, E>& twr,
-                                                           const float2* tab) {
+// Transform the fetched field (the next one starts loading); epi consumes
+// the result in the last stage's registers (output e of this thread is
+// element t + e N/E of its column), so nothing is written back to LDS.
+template <int N, int E, int C, typename EPI>
+__device__ __forceinline__ void hcol_transform_pp(HColPipe<N, E, C>& pp, float2* lds, const float2* next, int t,
+                                                  int cc, const FftTw<N, E>& twr, const float2* tab,
+                                                  const EPI& epi) {
   constexpr int LS = col_stride<N, C>();
   pp.put();
   if (next) pp.fetch(next);
   __syncthreads();
-  fresnel_1d_pp<N, E>(lds + cc * LS, lds + C * LS + cc * LS, t, twr, tab);
-  return lds + C * LS + cc * LS;
+  fresnel_1d_pp<N, E, kMaxRadix, false, EPI>(lds + cc * LS, lds + C * LS + cc * LS, t, twr, tab, epi);
 }
+
+// Consumers of the column transforms (output hooks, nothing stored).
+// x0 or y0 kept in registers:
+template <int E>
+struct HcolKeep {
+  static constexpr bool active = true, store = false;
+  float2 (&out)[E];
+  __device__ __forceinline__ float2 operator()(int e, float2 v) const {
+    out[e] = v;
+    return v;
+  }
+};
+
+// acc += (1 - d/|x0|) Re(conj(x0) w0):
+template <int E>
+struct HcolDot {
+  static constexpr bool active = true, store = false;
+  const float2 (&xv)[E];
+  const float (&dv)[E];
+  double& acc;
+  __device__ __forceinline__ float2 operator()(int e, float2 w) const {
+    const float am = sqrtf(xv[e].x * xv[e].x + xv[e].y * xv[e].y);
+    acc += (double)((1.0f - dv[e] / am) * (xv[e].x * w.x + xv[e].y * w.y));
+    return w;
+  }
+};
+
+// equal form, y0 = z0: acc += (1 - d/|x0|) |y0|^2 + d/|x0|^3 Re(conj(x0) y0)^2
+template <int E>
+struct HcolSame {
+  static constexpr bool active = true, store = false;
+  const float2 (&xv)[E];
+  const float (&dv)[E];
+  double& acc;
+  __device__ __forceinline__ float2 operator()(int e, float2 yv) const {
+    const float am = sqrtf(xv[e].x * xv[e].x + xv[e].y * xv[e].y);
+    const float xy = xv[e].x * yv.x + xv[e].y * yv.y;
+    const float t1 = (1.0f - dv[e] / am) * (yv.x * yv.x + yv.y * yv.y);
+    const float t2 = dv[e] / (am * am * am) * xy * xy;
+    acc += (double)(t1 + t2);
+    return yv;
+  }
+};
+
+// general form: acc += g Re(conj(y0) z0) + h Re(conj(x0) y0) Re(conj(x0) z0),
+// and for the pair acc2 += g |z0|^2 + h Re(conj(x0) z0)^2 (g = 1 - d/|x0|,
+// h = d/|x0|^3):
+template <int E, bool PAIR>
+struct HcolCross {
+  static constexpr bool active = true, store = false;
+  const float2 (&xv)[E];
+  const float2 (&yv)[E];
+  const float (&dv)[E];
+  double& acc;
+  double& acc2;
+  __device__ __forceinline__ float2 operator()(int e, float2 zv) const {
+    const float am = sqrtf(xv[e].x * xv[e].x + xv[e].y * xv[e].y);
+    const float g = 1.0f - dv[e] / am, h = dv[e] / (am * am * am);
+    const float xy = xv[e].x * yv[e].x + xv[e].y * yv[e].y;
+    const float xz = xv[e].x * zv.x + xv[e].y * zv.y;
+    acc += (double)(g * (yv[e].x * zv.x + yv[e].y * zv.y) + h * xy * xz);
+    if constexpr (PAIR) acc2 += (double)(g * (zv.x * zv.x + zv.y * zv.y) + h * xz * xz);
+    return zv;
+  }
+};
 
 template <int NT>
 __device__ __forceinline__ double block_sum_d(double v, double* red) {
@@ -305,55 +372,17 @@ __global__ __launch_bounds__(C*(N / E)) void bhh_cols_k(BhhArgs a, int tiles_per
   float dv[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) dv[e] = a.data[off + (int64_t)dct_nat<N>(t + e * T) * a.n + cc];
-  // x0 of this thread's elements
-  const float2* s = hcol_transform_pp<N, E, C>(pp, lds, a.F[3] + off, t, cc, twr, tab);
-  float2 xv[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) xv[e] = s[fpad(t + e * T)];
-  // (1 - d/|x0|) Re(conj(x0) w0), for each form
+  float2 xv[E];  // x0 of this thread's elements
+  hcol_transform_pp<N, E, C>(pp, lds, a.F[3] + off, t, cc, twr, tab, HcolKeep<E>{xv});
   double acc = 0.0, acc2 = 0.0;
-  s = hcol_transform_pp<N, E, C>(pp, lds, PAIR ? a.F[4] + off : a.F[1] + off, t, cc, twr, tab);
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const float2 w = s[fpad(t + e * T)];
-    const float am = sqrtf(xv[e].x * xv[e].x + xv[e].y * xv[e].y);
-    acc += (double)((1.0f - dv[e] / am) * (xv[e].x * w.x + xv[e].y * w.y));
-  }
-  if constexpr (PAIR) {
-    s = hcol_transform_pp<N, E, C>(pp, lds, a.F[1] + off, t, cc, twr, tab);
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const float2 w = s[fpad(t + e * T)];
-      const float am = sqrtf(xv[e].x * xv[e].x + xv[e].y * xv[e].y);
-      acc2 += (double)((1.0f - dv[e] / am) * (xv[e].x * w.x + xv[e].y * w.y));
-    }
-  }
-  s = hcol_transform_pp<N, E, C>(pp, lds, SAME ? nullptr : a.F[2] + off, t, cc, twr, tab);
+  hcol_transform_pp<N, E, C>(pp, lds, PAIR ? a.F[4] + off : a.F[1] + off, t, cc, twr, tab, HcolDot<E>{xv, dv, acc});
+  if constexpr (PAIR) hcol_transform_pp<N, E, C>(pp, lds, a.F[1] + off, t, cc, twr, tab, HcolDot<E>{xv, dv, acc2});
   if constexpr (SAME) {
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const float2 yv = s[fpad(t + e * T)];
-      const float am = sqrtf(xv[e].x * xv[e].x + xv[e].y * xv[e].y);
-      const float xy = xv[e].x * yv.x + xv[e].y * yv.y;
-      const float t1 = (1.0f - dv[e] / am) * (yv.x * yv.x + yv.y * yv.y);
-      const float t2 = dv[e] / (am * am * am) * xy * xy;
-      acc += (double)(t1 + t2);
-    }
+    hcol_transform_pp<N, E, C>(pp, lds, nullptr, t, cc, twr, tab, HcolSame<E>{xv, dv, acc});
   } else {
     float2 yv[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) yv[e] = s[fpad(t + e * T)];
-    s = hcol_transform_pp<N, E, C>(pp, lds, nullptr, t, cc, twr, tab);
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const float2 zv = s[fpad(t + e * T)];
-      const float am = sqrtf(xv[e].x * xv[e].x + xv[e].y * xv[e].y);
-      const float g = 1.0f - dv[e] / am, h = dv[e] / (am * am * am);
-      const float xy = xv[e].x * yv[e].x + xv[e].y * yv[e].y;
-      const float xz = xv[e].x * zv.x + xv[e].y * zv.y;
-      acc += (double)(g * (yv[e].x * zv.x + yv[e].y * zv.y) + h * xy * xz);
-      if constexpr (PAIR) acc2 += (double)(g * (zv.x * zv.x + zv.y * zv.y) + h * xz * xz);
-    }
+    hcol_transform_pp<N, E, C>(pp, lds, a.F[2] + off, t, cc, twr, tab, HcolKeep<E>{yv});
+    hcol_transform_pp<N, E, C>(pp, lds, nullptr, t, cc, twr, tab, HcolCross<E, PAIR>{xv, yv, dv, acc, acc2});
   }
   const double tot = block_sum_d<NT>(acc, red);
   if constexpr (PAIR) {

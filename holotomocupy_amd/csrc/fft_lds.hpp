@@ -235,12 +235,25 @@ __device__ __forceinline__ void tw_powers(float2 w, float2* p) {
   }
 }
 
+// Output hook of a transform's last stage.  There a thread's outputs are the
+// elements t + e N/E, e = b + r E/R for butterfly b and output r, so a
+// pointwise step on the transform's result can run in registers as the stage
+// writes it: epi(e, v) returns the value stored for element t + e N/E.  A
+// hook whose `store` is false consumes the results itself and the stage
+// writes nothing (its closing ordering point stays).
+struct NoEpi {
+  static constexpr bool active = false, store = true;
+  __device__ __forceinline__ float2 operator()(int, float2 v) const { return v; }
+};
+
 // One Stockham stage: span NS (product of radices done so far), radix R,
 // reading src and writing dst.  In place (src == dst) the reads of every
 // thread must finish before any write (MID: a second ordering point); with two
-// buffers the one ordering point after the writes is enough.
-template <int N, int E, int R, int NS, bool INV, bool WAVE = false, bool MID = true>
-__device__ __forceinline__ void stockham_stage(const float2* src, float2* dst, int t, const float2* w) {
+// buffers the one ordering point after the writes is enough.  EPI: the output
+// hook, last stage only.
+template <int N, int E, int R, int NS, bool INV, bool WAVE = false, bool MID = true, typename EPI = NoEpi>
+__device__ __forceinline__ void stockham_stage(const float2* src, float2* dst, int t, const float2* w,
+                                               const EPI& epi = EPI()) {
   constexpr int T = N / E;
   constexpr int NB = E / R;
   float2 a[NB][R];
@@ -270,14 +283,21 @@ __device__ __forceinline__ void stockham_stage(const float2* src, float2* dst, i
       for (int r = 1; r < R; ++r) a[b][r] = cmul(a[b][r], p[r]);
     }
     dft<R, INV>(a[b]);
-    const int base = (j - k) * R + k;
-    if constexpr (NS % 16 == 0) {
-      const int pb = fpad(base);
+    if constexpr (EPI::active) {
+      static_assert(NS * R == N, "the output hook runs on the last stage");
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst[pb + r * (NS / 16) * 17] = a[b][r];
-    } else {
+      for (int r = 0; r < R; ++r) a[b][r] = epi(b + r * NB, a[b][r]);
+    }
+    if constexpr (EPI::store) {
+      const int base = (j - k) * R + k;
+      if constexpr (NS % 16 == 0) {
+        const int pb = fpad(base);
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst[fpad(base + r * NS)] = a[b][r];
+        for (int r = 0; r < R; ++r) dst[pb + r * (NS / 16) * 17] = a[b][r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[fpad(base + r * NS)] = a[b][r];
+      }
     }
   }
   lds_sync<WAVE>();
@@ -292,13 +312,18 @@ __device__ __forceinline__ void stockham(float2* s, int t, const float2* w) {
   }
 }
 
-// Ping-pong stages: stage i reads buffer i % 2 of (a, b) and writes the other.
-template <int N, int E, int NS, int OFF, bool INV, int RM, bool WAVE, bool FLIP>
-__device__ __forceinline__ void stockham_pp(float2* a, float2* b, int t, const float2* w) {
+// Ping-pong stages: stage i reads buffer i % 2 of (a, b) and writes the other;
+// the last stage applies the output hook epi.
+template <int N, int E, int NS, int OFF, bool INV, int RM, bool WAVE, bool FLIP, typename EPI = NoEpi>
+__device__ __forceinline__ void stockham_pp(float2* a, float2* b, int t, const float2* w, const EPI& epi = EPI()) {
   if constexpr (NS < N) {
     constexpr int R = stage_radix<N, E, NS, RM>();
-    stockham_stage<N, E, R, NS, INV, WAVE, false>(FLIP ? b : a, FLIP ? a : b, t, w + OFF);
-    stockham_pp<N, E, NS * R, OFF + (NS > 1 ? E / R : 0), INV, RM, WAVE, !FLIP>(a, b, t, w);
+    if constexpr (NS * R >= N) {
+      stockham_stage<N, E, R, NS, INV, WAVE, false, EPI>(FLIP ? b : a, FLIP ? a : b, t, w + OFF, epi);
+    } else {
+      stockham_stage<N, E, R, NS, INV, WAVE, false>(FLIP ? b : a, FLIP ? a : b, t, w + OFF);
+      stockham_pp<N, E, NS * R, OFF + (NS > 1 ? E / R : 0), INV, RM, WAVE, !FLIP, EPI>(a, b, t, w, epi);
+    }
   }
 }
 
@@ -326,12 +351,14 @@ __device__ __forceinline__ void fft_lds(float2* s, int t, const FftTw<N, E, RM>&
 
 // The same transform of a with b as the second buffer: one ordering point per
 // stage instead of two.  The result is in b when fft_stages is odd, else in a.
-template <int N, int E, bool INV, int RM = kMaxRadix, bool WAVE = false>
-__device__ __forceinline__ void fft_lds_pp(float2* a, float2* b, int t, const FftTw<N, E, RM>& tw) {
+// epi: output hook of the last stage (see NoEpi).
+template <int N, int E, bool INV, int RM = kMaxRadix, bool WAVE = false, typename EPI = NoEpi>
+__device__ __forceinline__ void fft_lds_pp(float2* a, float2* b, int t, const FftTw<N, E, RM>& tw,
+                                           const EPI& epi = EPI()) {
   static_assert((N & (N - 1)) == 0 && N >= 2, "N must be a power of two");
   static_assert(E <= N && (E == 2 || E == 4 || E == 8 || E == 16 || E == 32), "E must be 2..32");
   static_assert(!WAVE || N / E == 64, "a wavefront-private transform takes exactly 64 threads");
-  stockham_pp<N, E, 1, 0, INV, RM, WAVE, false>(a, b, t, tw.w);
+  stockham_pp<N, E, 1, 0, INV, RM, WAVE, false, EPI>(a, b, t, tw.w, epi);
 }
 
 // Makhoul reordering that turns a length-N DCT-II into a length-N FFT:

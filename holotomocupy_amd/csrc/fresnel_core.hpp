@@ -71,18 +71,20 @@ __device__ __forceinline__ void dct_pair_map_pp(const float2* src, float2* dst, 
 
 // fresnel_1d of a with b as the second buffer (one ordering point per FFT
 // stage): the transform, the pair map and the inverse move the data 2 S + 1
-// times (S stages each), an odd count, so the result is always in b.
-template <int N, int E, int RM = kMaxRadix, bool WAVE = false>
+// times (S stages each), an odd count, so the result is always in b.  epi:
+// output hook of the inverse's last stage (a pointwise step on the result,
+// see NoEpi).
+template <int N, int E, int RM = kMaxRadix, bool WAVE = false, typename EPI = NoEpi>
 __device__ __forceinline__ void fresnel_1d_pp(float2* a, float2* b, int t, const FftTw<N, E, RM>& tw,
-                                              const float2* __restrict__ tab) {
+                                              const float2* __restrict__ tab, const EPI& epi = EPI()) {
   constexpr bool odd = fft_stages<N, E, RM>() % 2 == 1;
   fft_lds_pp<N, E, false, RM, WAVE>(a, b, t, tw);  // -> odd ? b : a
   if constexpr (odd) {
     dct_pair_map_pp<N, E, WAVE>(b, a, t, tab);
-    fft_lds_pp<N, E, true, RM, WAVE>(a, b, t, tw);  // a -> b
+    fft_lds_pp<N, E, true, RM, WAVE, EPI>(a, b, t, tw, epi);  // a -> b
   } else {
     dct_pair_map_pp<N, E, WAVE>(a, b, t, tab);
-    fft_lds_pp<N, E, true, RM, WAVE>(b, a, t, tw);  // b -> b (even count)
+    fft_lds_pp<N, E, true, RM, WAVE, EPI>(b, a, t, tw, epi);  // b -> b (even count)
   }
 }
 
