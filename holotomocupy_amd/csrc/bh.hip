@@ -212,16 +212,10 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(4))
       }
     }
     __syncthreads();
-    fresnel_1d<N, E>(s, t, twr, tab);
-    if (valid) {
-      float2* dst = a.T + (bj * a.nz + ty) * (int64_t)N;
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int i = t + e * T;
-        dst[i] = s[row_slot<N, E>(t, e)];
-      }
-    }
-    __syncthreads();
+    // the last stage writes the row to T in natural order from registers
+    fresnel_1d<N, E, kMaxRadix, false, LineStore<N>>(
+        s, t, twr, tab, LineStore<N>{valid ? a.T + (bj * a.nz + ty) * (int64_t)N : nullptr, t, T});
+    __syncthreads();  // the next angle's gather writes s
   }
 }
 
@@ -347,11 +341,11 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
     asm volatile("" : "+v"(t));
     const int64_t bj = b * a.ndist + j;
     float2* row = a.T + (bj * a.nz + ty) * (int64_t)N;
+    // the row's FFT slots t + e T (natural elements dct_nat of them) straight
+    // into registers for the first transform stage
+    float2 h[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int i = t + e * T;
-      s[row_slot<N, E>(t, e)] = valid ? row[i] : make_float2(0.f, 0.f);
-    }
+    for (int e = 0; e < E; ++e) h[e] = valid ? row[dct_nat<N>(t + e * T)] : make_float2(0.f, 0.f);
     const float mb = a.mag[bj], ry = a.pos[2 * bj], rx = a.pos[2 * bj + 1];
     const BsPos py = bs_pos(ty, a.nz, a.nzpsi, mb, ry);
     const int iy = py.i - 1;
@@ -366,8 +360,7 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
       row_span<N>(a.npsi, mb, rx, lo, hi);
       stage_row<TO, T, true>(cb, a.npsi, a.nzpsi, iy, wy, dwy, lo, hi, t, cy, cdy);
     }
-    __syncthreads();
-    fresnel_1d<N, E>(s, t, twr, tab);
+    fresnel_1d_from<N, E>(h, s, t, twr, tab);  // its first ordering point also orders the staged rows
     const int p = a.pidx ? a.pidx[b] : 0;
     if (p != pcur) {
       flush(pcur);

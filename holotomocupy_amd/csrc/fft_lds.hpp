@@ -269,7 +269,7 @@ __device__ __forceinline__ void stockham_stage(const float2* src, float2* dst, i
       for (int r = 0; r < R; ++r) a[b][r] = src[fpad(j + r * (N / R))];
     }
   }
-  if constexpr (MID) lds_sync<WAVE>();
+  if constexpr (MID && EPI::store) lds_sync<WAVE>();  // a stage that writes nothing reads only
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const int j = t + b * T;
@@ -303,12 +303,17 @@ __device__ __forceinline__ void stockham_stage(const float2* src, float2* dst, i
   lds_sync<WAVE>();
 }
 
-template <int N, int E, int NS, int OFF, bool INV, int RM, bool WAVE>
-__device__ __forceinline__ void stockham(float2* s, int t, const float2* w) {
+// In-place stages from span NS; the last stage applies the output hook epi.
+template <int N, int E, int NS, int OFF, bool INV, int RM, bool WAVE, typename EPI = NoEpi>
+__device__ __forceinline__ void stockham(float2* s, int t, const float2* w, const EPI& epi = EPI()) {
   if constexpr (NS < N) {
     constexpr int R = stage_radix<N, E, NS, RM>();
-    stockham_stage<N, E, R, NS, INV, WAVE>(s, s, t, w + OFF);
-    stockham<N, E, NS * R, OFF + (NS > 1 ? E / R : 0), INV, RM, WAVE>(s, t, w);
+    if constexpr (NS * R >= N) {
+      stockham_stage<N, E, R, NS, INV, WAVE, true, EPI>(s, s, t, w + OFF, epi);
+    } else {
+      stockham_stage<N, E, R, NS, INV, WAVE>(s, s, t, w + OFF);
+      stockham<N, E, NS * R, OFF + (NS > 1 ? E / R : 0), INV, RM, WAVE, EPI>(s, t, w, epi);
+    }
   }
 }
 
@@ -327,6 +332,27 @@ __device__ __forceinline__ void stockham_pp(float2* a, float2* b, int t, const f
   }
 }
 
+// First stage (span 1) of a transform whose inputs the thread holds in
+// registers: h[e] is element t + e N/E, the elements the stage would read
+// (butterfly b, input r: e = b + r E/R).  Writes dst, then one ordering point;
+// nothing is read from dst, so in place needs no second one.
+template <int N, int E, bool INV, int RM, bool WAVE>
+__device__ __forceinline__ void stockham_first_from(const float2 (&h)[E], float2* dst, int t) {
+  constexpr int R = stage_radix<N, E, 1, RM>();
+  constexpr int T = N / E, NB = E / R;
+  float2 a[NB][R];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) a[b][r] = h[b + r * NB];
+    dft<R, INV>(a[b]);
+    const int j = t + b * T;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[fpad(j * R + r)] = a[b][r];
+  }
+  lds_sync<WAVE>();
+}
+
 // Number of Stockham stages of a length-N transform.
 template <int N, int E, int RM = kMaxRadix, int NS = 1>
 constexpr int fft_stages() {
@@ -341,12 +367,12 @@ constexpr int fft_stages() {
 // with WAVE, every thread of the wavefront that owns the buffer, which must
 // be all N/E threads of the transform).  t in [0, N/E) is the thread's index
 // within this FFT and tw its registers from fft_load_twiddles with the same t.
-template <int N, int E, bool INV, int RM = kMaxRadix, bool WAVE = false>
-__device__ __forceinline__ void fft_lds(float2* s, int t, const FftTw<N, E, RM>& tw) {
+template <int N, int E, bool INV, int RM = kMaxRadix, bool WAVE = false, typename EPI = NoEpi>
+__device__ __forceinline__ void fft_lds(float2* s, int t, const FftTw<N, E, RM>& tw, const EPI& epi = EPI()) {
   static_assert((N & (N - 1)) == 0 && N >= 2, "N must be a power of two");
   static_assert(E <= N && (E == 2 || E == 4 || E == 8 || E == 16 || E == 32), "E must be 2..32");
   static_assert(!WAVE || N / E == 64, "a wavefront-private transform takes exactly 64 threads");
-  stockham<N, E, 1, 0, INV, RM, WAVE>(s, t, tw.w);
+  stockham<N, E, 1, 0, INV, RM, WAVE, EPI>(s, t, tw.w, epi);
 }
 
 // The same transform of a with b as the second buffer: one ordering point per

@@ -41,21 +41,14 @@ __global__ __launch_bounds__(kRowThreads) void fresnel_rows_k(const float2* in, 
   const float2* src = in + fb * in_stride + fy * N;
   FftTw<N, E> twr;
   fft_load_twiddles<N, E>(twr, t, tw);
+  // FFT slots t + e T of the row (natural elements dct_nat of them) straight
+  // into registers for the first transform stage
+  float2 h[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int i = t + e * T;
-    s[row_slot<N, E>(t, e)] = valid ? src[i] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  fresnel_1d<N, E>(s, t, twr, tab);
-  if (valid) {
-    float2* dst = out + fb * out_stride + fy * N;
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int i = t + e * T;
-      dst[i] = s[row_slot<N, E>(t, e)];
-    }
-  }
+  for (int e = 0; e < E; ++e) h[e] = valid ? src[dct_nat<N>(t + e * T)] : make_float2(0.f, 0.f);
+  // the last stage writes the row to out in natural order from registers
+  fresnel_1d_from<N, E, kMaxRadix, false, LineStore<N>>(
+      h, s, t, twr, tab, LineStore<N>{valid ? out + fb * out_stride + fy * N : nullptr, t, T});
 }
 
 // Columns of length N (= nz) of a [batch, N, n] field, C adjacent columns per

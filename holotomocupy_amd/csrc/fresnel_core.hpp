@@ -37,13 +37,40 @@ __device__ __forceinline__ void dct_pair_map(float2* s, int t, const float2* __r
   lds_sync<WAVE>();
 }
 
-template <int N, int E, int RM = kMaxRadix, bool WAVE = false>
+// epi: output hook of the inverse's last stage (see NoEpi).
+template <int N, int E, int RM = kMaxRadix, bool WAVE = false, typename EPI = NoEpi>
 __device__ __forceinline__ void fresnel_1d(float2* s, int t, const FftTw<N, E, RM>& tw,
-                                           const float2* __restrict__ tab) {
+                                           const float2* __restrict__ tab, const EPI& epi = EPI()) {
   fft_lds<N, E, false, RM, WAVE>(s, t, tw);
   dct_pair_map<N, E, WAVE>(s, t, tab);
-  fft_lds<N, E, true, RM, WAVE>(s, t, tw);
+  fft_lds<N, E, true, RM, WAVE, EPI>(s, t, tw, epi);
 }
+
+// fresnel_1d of a line whose FFT inputs the thread holds: h[e] is FFT slot
+// t + e N/E (the forward transform's first stage runs on them in registers,
+// so the line is never written to s in slot order first).
+template <int N, int E, int RM = kMaxRadix, bool WAVE = false, typename EPI = NoEpi>
+__device__ __forceinline__ void fresnel_1d_from(const float2 (&h)[E], float2* s, int t, const FftTw<N, E, RM>& tw,
+                                                const float2* __restrict__ tab, const EPI& epi = EPI()) {
+  constexpr int R0 = stage_radix<N, E, 1, RM>();
+  stockham_first_from<N, E, false, RM, WAVE>(h, s, t);
+  stockham<N, E, R0, 0, false, RM, WAVE>(s, t, tw.w);
+  dct_pair_map<N, E, WAVE>(s, t, tab);
+  fft_lds<N, E, true, RM, WAVE, EPI>(s, t, tw, epi);
+}
+
+// Output hook that writes a line's result to global memory in natural order:
+// element t + e N/E of the transform is natural element dct_nat of it.
+template <int N>
+struct LineStore {
+  static constexpr bool active = true, store = false, sync_after = false;
+  float2* dst;  // the line, natural order; nullptr: not stored
+  int t, T;
+  __device__ __forceinline__ float2 operator()(int e, float2 v) const {
+    if (dst) dst[dct_nat<N>(t + e * T)] = v;
+    return v;
+  }
+};
 
 // The pair map from src to dst (each thread reads and writes its own pairs).
 template <int N, int E, bool WAVE = false>
