@@ -197,22 +197,23 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(4))
       __syncthreads();
 #pragma unroll 2
       for (int e = 0; e < E; ++e) {
-        const int i = t + e * T;
+        const int i = dct_nat<N>(t + e * T);  // the pixel of the thread's FFT slot t + e T
         TO v, d1, d2;
         row_gather<TO, false>(cy, cy, lo, hi, bs_pos(i, N, a.npsi, mb, rx), v, d1, d2);
-        s[row_slot<N, E>(t, e)] = cmul(prow[i], expi_fast(to_c(v)));
+        s[fpad(t + e * T)] = cmul(prow[i], expi_fast(to_c(v)));
       }
     } else {
 #pragma unroll 1
       for (int e = 0; e < E; ++e) {
-        const int i = t + e * T;
+        const int i = dct_nat<N>(t + e * T);
         TO v, d1, d2;
         s_gather<TO, false>(cb, a.npsi, a.nzpsi, bs_pos(i, N, a.npsi, mb, rx), iy, wy, wy, v, d1, d2);
-        s[row_slot<N, E>(t, e)] = cmul(prow[i], expi_fast(to_c(v)));
+        s[fpad(t + e * T)] = cmul(prow[i], expi_fast(to_c(v)));
       }
     }
-    __syncthreads();
-    // the last stage writes the row to T in natural order from registers
+    // the transform's first stage reads only the slots t + e T this thread
+    // wrote, so no ordering point is needed before it; the last stage writes
+    // the row to T in natural order from registers
     fresnel_1d<N, E, kMaxRadix, false, LineStore<N>>(
         s, t, twr, tab, LineStore<N>{valid ? a.T + (bj * a.nz + ty) * (int64_t)N : nullptr, t, T});
     __syncthreads();  // the next angle's gather writes s
@@ -320,7 +321,9 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
   fft_load_twiddles<N, E>(twr, t0, a.twx);
   const float2* tab = a.tabx + (int64_t)(j * 2 + 1) * 3 * (N / 2);
   const int64_t plane = (int64_t)a.nzpsi * a.npsi;
-  float2* acc = lacc + r * N;  // element i is only touched by thread t = i % T
+  // element dct_nat(t + e T) is accumulated by thread t; the zeroing and the
+  // flushes take elements t + e T, with workgroup barriers in between
+  float2* acc = lacc + r * N;
   for (int e = 0; e < E; ++e) acc[t0 + e * T] = make_float2(0.f, 0.f);
   int pcur = a.pidx ? a.pidx[0] : 0;
   auto flush = [&](int p) {
@@ -360,7 +363,10 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
       row_span<N>(a.npsi, mb, rx, lo, hi);
       stage_row<TO, T, true>(cb, a.npsi, a.nzpsi, iy, wy, dwy, lo, hi, t, cy, cdy);
     }
-    fresnel_1d_from<N, E>(h, s, t, twr, tab);  // its first ordering point also orders the staged rows
+    // its first ordering point also orders the staged rows; the epilogue reads
+    // back only this thread's outputs (elements t + e T), so the last stage
+    // needs no closing ordering point
+    fresnel_1d_from<N, E, kMaxRadix, false, OwnOutputs>(h, s, t, twr, tab);
     const int p = a.pidx ? a.pidx[b] : 0;
     if (p != pcur) {
       flush(pcur);
@@ -370,8 +376,8 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
     float s1 = 0.f, s2 = 0.f;
 #pragma unroll 1
     for (int e = 0; e < E; ++e) {
-      const int i = t + e * T;
-      const float2 y12 = s[row_slot<N, E>(t, e)];
+      const int i = dct_nat<N>(t + e * T);  // natural element of the thread's output e
+      const float2 y12 = s[fpad(t + e * T)];
       TO v, d1, d2;
       if constexpr (STAGED)
         row_gather<TO, true>(cy, cdy, lo, hi, bs_pos(i, N, a.npsi, mb, rx), v, d1, d2);

@@ -59,6 +59,14 @@ __device__ __forceinline__ void fresnel_1d_from(const float2 (&h)[E], float2* s,
   fft_lds<N, E, true, RM, WAVE, EPI>(s, t, tw, epi);
 }
 
+// Output "hook" that only drops the last stage's closing ordering point, for
+// a caller that reads back nothing but its own outputs (element t + e N/E at
+// s[fpad(t + e N/E)]): a thread's LDS accesses complete in order.
+struct OwnOutputs {
+  static constexpr bool active = false, store = true, sync_after = false;
+  __device__ __forceinline__ float2 operator()(int, float2 v) const { return v; }
+};
+
 // Output hook that writes a line's result to global memory in natural order:
 // element t + e N/E of the transform is natural element dct_nat of it.
 template <int N>
