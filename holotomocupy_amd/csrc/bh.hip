@@ -228,7 +228,7 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(4))
 // E objective terms are summed in FP32 and added to the FP64 total once.
 template <int E>
 struct BhResidual {
-  static constexpr bool active = true, store = true;
+  static constexpr bool active = true, store = true, sync_after = true;
   const float (&dv)[E];
   float sc;
   float& objf;

@@ -264,7 +264,7 @@ __device__ __forceinline__ void hcol_transform_pp(HColPipe<N, E, C>& pp, float2*
 // x0 or y0 kept in registers:
 template <int E>
 struct HcolKeep {
-  static constexpr bool active = true, store = false;
+  static constexpr bool active = true, store = false, sync_after = true;
   float2 (&out)[E];
   __device__ __forceinline__ float2 operator()(int e, float2 v) const {
     out[e] = v;
@@ -275,7 +275,7 @@ struct HcolKeep {
 // acc += (1 - d/|x0|) Re(conj(x0) w0):
 template <int E>
 struct HcolDot {
-  static constexpr bool active = true, store = false;
+  static constexpr bool active = true, store = false, sync_after = true;
   const float2 (&xv)[E];
   const float (&dv)[E];
   double& acc;
@@ -289,7 +289,7 @@ struct HcolDot {
 // equal form, y0 = z0: acc += (1 - d/|x0|) |y0|^2 + d/|x0|^3 Re(conj(x0) y0)^2
 template <int E>
 struct HcolSame {
-  static constexpr bool active = true, store = false;
+  static constexpr bool active = true, store = false, sync_after = true;
   const float2 (&xv)[E];
   const float (&dv)[E];
   double& acc;
@@ -308,7 +308,7 @@ struct HcolSame {
 // h = d/|x0|^3):
 template <int E, bool PAIR>
 struct HcolCross {
-  static constexpr bool active = true, store = false;
+  static constexpr bool active = true, store = false, sync_after = true;
   const float2 (&xv)[E];
   const float2 (&yv)[E];
   const float (&dv)[E];

@@ -240,9 +240,11 @@ __device__ __forceinline__ void tw_powers(float2 w, float2* p) {
 // pointwise step on the transform's result can run in registers as the stage
 // writes it: epi(e, v) returns the value stored for element t + e N/E.  A
 // hook whose `store` is false consumes the results itself and the stage
-// writes nothing (its closing ordering point stays).
+// writes nothing.  `sync_after` false drops the stage's closing ordering
+// point, for callers that then read back only their own outputs or leave the
+// buffer alone until an ordering point of their own.
 struct NoEpi {
-  static constexpr bool active = false, store = true;
+  static constexpr bool active = false, store = true, sync_after = true;
   __device__ __forceinline__ float2 operator()(int, float2 v) const { return v; }
 };
 
@@ -300,7 +302,7 @@ __device__ __forceinline__ void stockham_stage(const float2* src, float2* dst, i
       }
     }
   }
-  lds_sync<WAVE>();
+  if constexpr (EPI::sync_after) lds_sync<WAVE>();
 }
 
 // In-place stages from span NS; the last stage applies the output hook epi.
