@@ -375,7 +375,8 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
     const float2* prow = (a.prb_p ? a.prb_p[p] : a.prb + (int64_t)p * a.ndist * a.nz * N) + ((int64_t)j * a.nz + ty) * N;
     float s1 = 0.f, s2 = 0.f;
 #pragma unroll 1
-    for (int e = 0; e < E; ++e) {
+    for (int k = 0; k < E; ++k) {
+      const int e = pair_order<E>(k);
       const int i = dct_nat<N>(t + e * T);  // natural element of the thread's output e
       const float2 y12 = s[fpad(t + e * T)];
       TO v, d1, d2;

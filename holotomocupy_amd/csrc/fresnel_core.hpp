@@ -123,6 +123,18 @@ __device__ __forceinline__ void fresnel_1d_pp(float2* a, float2* b, int t, const
   }
 }
 
+// Order in which a row-pass thread visits its E slots t + e N/E when it
+// writes their natural elements dct_nat(t + e N/E) to HBM: 0, E-1, 1, E-2, ...
+// Natural elements 2m and 2m + 1 share a cache line and belong to slots e and
+// E-1-e of threads t and N/E-1-t, so in this order the two halves of a line
+// are written in consecutive iterations instead of E-1 apart, and lines are
+// not written back half-filled (K3: 2722 -> 2178 MiB written per 16-angle
+// launch at 2048^2, the algorithmic figure).
+template <int E>
+__device__ __forceinline__ int pair_order(int k) {
+  return (k & 1) ? E - 1 - (k >> 1) : (k >> 1);
+}
+
 // Column-pass elements per thread: 32 at N = 2048 (64 threads per column, so a
 // 4-column tile is one 256-thread workgroup), else 8.
 template <int N>
