@@ -175,7 +175,7 @@ STAGES = ["prefilter", "rows_fwd", "cols", "rows_adj", "sadj", "prefilter_grad",
 # FETCH_SIZE and WRITE_SIZE runs scaled by what the counters read for a 256 MiB
 # device-to-device copy (reads x2.00, writes x1.00); the commands and per-kernel
 # figures are in profiles/bh_gradient_fused_2048_d4_a16_traffic.md.
-TRAFFIC_MIB = {"bh_fwd_rows_k": 986.4 + 2048.0, "bh_cols_k": 3113.6 + 3313.5, "bh_adj_rows_k": 3993.0 + 2177.3}
+TRAFFIC_MIB = {"bh_fwd_rows_k": 1503.4 + 2048.0, "bh_cols_k": 3149.4 + 3399.1, "bh_adj_rows_k": 4037.8 + 2177.6}
 TRAFFIC_WORKLOAD = (2048, 4, 1024, 16)
 
 
