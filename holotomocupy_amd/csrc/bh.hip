@@ -321,10 +321,10 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
   fft_load_twiddles<N, E>(twr, t0, a.twx);
   const float2* tab = a.tabx + (int64_t)(j * 2 + 1) * 3 * (N / 2);
   const int64_t plane = (int64_t)a.nzpsi * a.npsi;
-  // element dct_nat(t + e T) is accumulated by thread t; the zeroing and the
-  // flushes take elements t + e T, with workgroup barriers in between
+  // element dct_nat(t + e T) belongs to thread t in the zeroing, the
+  // epilogue and the flushes alike, so no barrier orders them
   float2* acc = lacc + r * N;
-  for (int e = 0; e < E; ++e) acc[t0 + e * T] = make_float2(0.f, 0.f);
+  for (int e = 0; e < E; ++e) acc[dct_nat<N>(t0 + e * T)] = make_float2(0.f, 0.f);
   int pcur = a.pidx ? a.pidx[0] : 0;
   auto flush = [&](int p) {
     if (!valid) return;
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
     const int t = t0;
     const float rho = a.rho_prb[p];
     for (int e = 0; e < E; ++e) {
-      const int i = t + e * T;
+      const int i = dct_nat<N>(t + e * T);
       float2 v = cscale(acc[i], rho);
       if (a.accumulate_prb) v = cadd(g[i], v);
       g[i] = v;
