@@ -374,10 +374,14 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
     }
     const float2* prow = (a.prb_p ? a.prb_p[p] : a.prb + (int64_t)p * a.ndist * a.nz * N) + ((int64_t)j * a.nz + ty) * N;
     float s1 = 0.f, s2 = 0.f;
+    // the probe value of the next iteration is loaded one iteration ahead
+    float2 pnext = prow[dct_nat<N>(t + pair_order<E>(0) * T)];
 #pragma unroll 1
     for (int k = 0; k < E; ++k) {
       const int e = pair_order<E>(k);
       const int i = dct_nat<N>(t + e * T);  // natural element of the thread's output e
+      const float2 pv = pnext;
+      if (k + 1 < E) pnext = prow[dct_nat<N>(t + pair_order<E>(k + 1) * T)];
       const float2 y12 = s[fpad(t + e * T)];
       TO v, d1, d2;
       if constexpr (STAGED)
@@ -386,7 +390,7 @@ __global__ __launch_bounds__(kRowThreads) void bh_adj_rows_k(BhArgs a) {
         s_gather<TO, true>(cb, a.npsi, a.nzpsi, bs_pos(i, N, a.npsi, mb, rx), iy, wy, dwy, v, d1, d2);
       const float2 ex = expi_fast(to_c(v));
       acc[i] = cadd(acc[i], cmulc(y12, ex));                // y12 conj(x12)
-      const float2 q = cmulc(cmulc(y12, prow[i]), ex);      // y12 conj(prb) conj(x12)
+      const float2 q = cmulc(cmulc(y12, pv), ex);           // y12 conj(prb) conj(x12)
       const TO g = from_c<TO>(make_float2(q.y, -q.x));      // * (-i)
       s1 += vredot(g, d1);
       s2 += vredot(g, d2);
