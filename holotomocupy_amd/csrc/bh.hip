@@ -195,12 +195,16 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(4))
       row_span<N>(a.npsi, mb, rx, lo, hi);
       stage_row<TO, T, false>(cb, a.npsi, a.nzpsi, iy, wy, wy, lo, hi, t, cy, cy);
       __syncthreads();
+      // the probe value of the next iteration is loaded one iteration ahead
+      float2 pnext = prow[dct_nat<N>(t)];
 #pragma unroll 2
       for (int e = 0; e < E; ++e) {
         const int i = dct_nat<N>(t + e * T);  // the pixel of the thread's FFT slot t + e T
+        const float2 pv = pnext;
+        if (e + 1 < E) pnext = prow[dct_nat<N>(t + (e + 1) * T)];
         TO v, d1, d2;
         row_gather<TO, false>(cy, cy, lo, hi, bs_pos(i, N, a.npsi, mb, rx), v, d1, d2);
-        s[fpad(t + e * T)] = cmul(prow[i], expi_fast(to_c(v)));
+        s[fpad(t + e * T)] = cmul(pv, expi_fast(to_c(v)));
       }
     } else {
 #pragma unroll 1
