@@ -158,9 +158,18 @@ __global__ __launch_bounds__(kRowThreads) void bhh_chain_k(BhhArgs a, int rpb) {
     }
     __syncthreads();
     const int64_t orow = (bj * a.nz + ty) * (int64_t)N;
+    // the probe values of the next iteration are loaded one iteration ahead
+    float2 pn = a.prb[prow + t], pyn = a.py[prow + t];
+    float2 pzn = SAME ? make_float2(0.f, 0.f) : a.pz[prow + t];
 #pragma unroll 2
     for (int e = 0; e < E; ++e) {
       const int i = t + e * T;
+      const float2 p = pn, pyv = pyn, pzv = pzn;
+      if (e + 1 < E) {
+        pn = a.prb[prow + i + T];
+        pyn = a.py[prow + i + T];
+        if constexpr (!SAME) pzn = a.pz[prow + i + T];
+      }
       const BsPos px = bs_pos(i, N, a.npsi, mb, rx);
       const int ix = px.i - 1;
       float wx[4], dwx[4], d2wx[4];
@@ -191,7 +200,6 @@ __global__ __launch_bounds__(kRowThreads) void bhh_chain_k(BhhArgs a, int rpb) {
       const float2 wv = to_c(vfma(yyx * zzx, sxx, vfma(1.0f, w1, w0)));
       const float2 ex = expi_fast(to_c(x22));
       const float2 y12 = cmul(ex, mul_i(yv));
-      const float2 p = a.prb[prow + i], pyv = a.py[prow + i];
       if (valid) {
         a.F[0][orow + i] = cmul(p, ex);
         a.F[1][orow + i] = cadd(cmul(pyv, ex), cmul(p, y12));
@@ -200,7 +208,6 @@ __global__ __launch_bounds__(kRowThreads) void bhh_chain_k(BhhArgs a, int rpb) {
           const float2 w12 = cmul(ex, make_float2(-wv.y - yz.x, wv.x - yz.y));
           a.F[3][orow + i] = cadd(cscale(cmul(pyv, y12), 2.0f), cmul(p, w12));
         } else {
-          const float2 pzv = a.pz[prow + i];
           const float2 zv = to_c(vfma(-zzx, sx, az));
           const float2 z12 = cmul(ex, mul_i(zv));
           const float2 yz = cmul(yv, zv);
