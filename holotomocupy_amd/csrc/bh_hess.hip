@@ -121,6 +121,8 @@ __global__ __launch_bounds__(kRowThreads) void bhh_chain_k(BhhArgs a, int rpb) {
     const TO* czb = (const TO*)a.cz + b * plane;
     int lo, hi;
     row_span<N>(a.npsi, mb, rx, lo, hi);
+    // two columns in flight per thread: their loads overlap
+#pragma unroll 2
     for (int x = lo + t; x < hi; x += T) {
       TO r0 = vzero<TO>(), r1 = vzero<TO>(), r2 = vzero<TO>();
       TO y0 = vzero<TO>(), y1 = vzero<TO>(), z0 = vzero<TO>(), z1 = vzero<TO>();
